@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark of the CNS_NoSpec RK3 time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--edge 512]
+
+Headline (``value``): the plain FP64 path through the C ABI (``cns_advance``:
+ghost fill, ctoprim + Courant reduction, diffterm, hypterm, RK update, three
+stages, dt synchronised to the host every step as in the reference) on one
+periodic 512^3 box (520^3 ghosted, ng = 4), seeded sinusoidal state with
+random phases on wavenumbers 1-4.  A step is one RK3 step of the whole box;
+value = interior cells x steps / wall time of the timed steps (inputs resident
+in HBM).  The other workloads (64^3 plain, batched, served) are reported
+under ``secondary``.
+
+``roofline``: the dominant kernel's algorithmic HBM bytes per launch (each
+ABI array element read once over its needed extent, each output written once)
+divided by its average duration, measured with HIP events around every launch
+inside the timed region on the launch stream.  ``traffic`` is the measured HBM
+bytes per launch from rocprofv3 PMC counters (profiles/hbm_traffic.txt) when
+present.  ``cpu_baseline``: the reference cpu/ path itself (oracle/_ref,
+compiled from the reference sources) timed on one host core on a bounded
+sample, or the oracle restatement where that build is absent.
+"""
+from __future__ import annotations
+
+import argparse
+import contextlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-steps/s (plain FP64 vs batched/FP32-flux) + p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+C2050_64_CELL_STEPS = 64 ** 3 * 100 / 2.7245  # cns_nospec.xls!Sheet1!A7:E7
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+
+
+def algorithmic_bytes(kind: str, n: int, stage: int = 2) -> float:
+    """Per-launch algorithmic HBM bytes of one ABI kernel on an n^3 box (f64):
+    each input element read once over its needed extent, each output written once."""
+    g3, n3 = (n + 8) ** 3, n ** 3
+    rk = 20 * n3 if stage == 1 else 25 * n3
+    table = {
+        "fill": 10 * (g3 - n3),
+        "ctoprim": 11 * g3,
+        "diffterm": 4 * g3 + 5 * n3,
+        "hypterm": 8 * g3 + 5 * n3,
+        "rk": rk,
+        "hypterm_rk": 8 * g3 + 5 * n3 + rk,
+        "fill_ctoprim": 10 * (g3 - n3) + 11 * g3,
+    }
+    return 8.0 * table[kind]
+
+
+def step_algorithmic_bytes(n: int) -> float:
+    return 8.0 * (99 * (n + 8) ** 3 + 70 * n ** 3)
+
+
+def read_traffic(kind: str, edge: int):
+    """Measured HBM bytes per launch for (kind, edge) from profiles/hbm_traffic.txt."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("#") or "=" not in line:
+            continue
+        kv = dict(tok.split("=", 1) for tok in line.split() if "=" in tok)
+        if kv.get("kernel") == kind and kv.get("edge") == str(edge) and "bytes" in kv:
+            return float(kv["bytes"])
+    return None
+
+
+def run_plain(edge: int, steps: int, warmup: int, seed: int = 2024, kmax: int = 4,
+              profile: bool = True):
+    import torch
+    from cns_nospec_amd import ops
+    from cns_nospec_amd.layout import BoxLayout
+    from cns_nospec_amd.state import device_sinusoid_state
+
+    dev = torch.device("cuda:0")
+    lay = BoxLayout.cube(edge)
+    U, Unew, Q, D, F = ops.allocate_variables(lay, dev)
+    device_sinusoid_state(lay, seed, dev, mach=0.085, kmax=kmax, out=U)
+    st = ops.Stepper(lay)
+    for _ in range(warmup):
+        st.new_advance(U, Unew, Q, D, F)
+    if profile:
+        st.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dts = [st.new_advance(U, Unew, Q, D, F) for _ in range(steps)]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kern = st.profile_read() if profile else {}
+    st.profile(False)
+    st.close()
+    del U, Unew, Q, D, F
+    torch.cuda.empty_cache()
+    return {"edge": edge, "elapsed_s": el, "cell_steps_per_s": edge ** 3 * steps / el,
+            "ms_per_step": 1e3 * el / steps, "dt_last": dts[-1], "kernels": kern}
+
+
+@contextlib.contextmanager
+def quiet_stdout():
+    """The reference prints `dt, courno` each step; keep it off our one JSON line."""
+    sys.stdout.flush()
+    saved = os.dup(1)
+    null = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(null, 1)
+    try:
+        yield
+    finally:
+        os.dup2(saved, 1)
+        os.close(null)
+        os.close(saved)
+
+
+def cpu_baseline(edge: int = 128, budget_s: float = 15.0):
+    """Reference cpu/ new_advance on one core, as many steps of an edge^3 box as fit
+    the time budget (at least 2)."""
+    import numpy as np
+    import oracle
+    from cns_nospec_amd.layout import NC, NQ, BoxLayout
+    from cns_nospec_amd.state import ALAM, CFL, ETA, box_dx, sinusoid_state
+
+    lay = BoxLayout.cube(edge)
+    U = sinusoid_state(lay, 7, kmax=4)
+    if oracle.reference_available():
+        kind = "reference"
+        ref = oracle.Reference()
+        ref.setup(lay.n, box_dx(edge), CFL, ETA, ALAM)
+        Ua = lay.ghosted_to_aos(U, NC)
+        Unew, Q = np.zeros_like(Ua), np.zeros(lay.dimg + (NQ,))
+        D, F = np.zeros(lay.n + (NC,)), np.zeros(lay.n + (NC,))
+        step = lambda: ref.new_advance(Ua, Unew, Q, D, F)
+    else:
+        kind = "port"
+        orc = oracle.Oracle()
+        Unew, Q = lay.new_ghosted(NC), lay.new_ghosted(NQ)
+        D, F = lay.new_interior(NC), lay.new_interior(NC)
+        dx = box_dx(edge)
+        step = lambda: orc.advance(lay, dx, CFL, ETA, ALAM, U, Unew, Q, D, F)
+    n = 0
+    with quiet_stdout():
+        t0 = time.perf_counter()
+        while n < 2 or (time.perf_counter() - t0) < budget_s:
+            step()
+            n += 1
+        el = time.perf_counter() - t0
+    return {"value": edge ** 3 * n / el, "unit": "cell-steps/s", "cores": 1, "kind": kind,
+            "sample": f"{n} RK3 steps of one {edge}^3 box (seeded sinusoids), single thread, "
+                      f"{el:.1f} s; the path has no threading (its OpenMP is commented out)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--edge", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): "
+              "independent boxes need no communication and a single box is not sharded here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from cns_nospec_amd import _lib
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    _lib.load()
+
+    main_run = run_plain(args.edge, args.steps, args.warmup)
+    kern = main_run["kernels"]
+    dom = max(kern, key=lambda k: kern[k][0]) if kern else None
+    roof = None
+    if dom:
+        ms_tot, cnt = kern[dom]
+        avg_s = ms_tot / cnt / 1e3
+        nbytes = algorithmic_bytes(dom, args.edge)
+        ach = nbytes / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(dom, args.edge),
+                "algorithmic_bytes": nbytes, "avg_ms": round(avg_s * 1e3, 4)}
+    step_bytes = step_algorithmic_bytes(args.edge)
+    out = {
+        "metric": METRIC,
+        "value": main_run["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_run["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain FP64 RK3 step through the C ABI, one periodic "
+                               f"{args.edge}^3 box ({args.edge + 8}^3 ghosted, ng=4), seeded "
+                               f"sinusoids on wavenumbers 1-4, dt to host every step",
+                   "edge": args.edge, "ng": 4, "cfl": 0.5, "eta": 1.8e-4, "alam": 150.0},
+        "step_hbm_fraction": round(step_bytes / (main_run["ms_per_step"] / 1e3) / 1e9
+                                   / HBM_PEAK_GBS, 4),
+        "roofline": roof,
+        "kernels_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
+    }
+    if not args.no_secondary:
+        sec = {}
+        r64 = run_plain(64, 100, 5, seed=64, kmax=2, profile=False)
+        sec["plain_64"] = {"value": r64["cell_steps_per_s"], "unit": "cell-steps/s",
+                           "ms_per_step": r64["ms_per_step"], "steps": 100,
+                           "vs_c2050_published": r64["cell_steps_per_s"] / C2050_64_CELL_STEPS}
+        out["secondary"] = sec
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,100 @@
+"""Loader of libcns_hip.so (the HIP library behind include/cns.h).
+
+There is no CPU fallback: if the library is missing or fails to load, every
+entry point raises.  Build it with ``python -m cns_nospec_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcns_hip.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "cns.h")
+
+CNS_OK, CNS_EINVAL, CNS_EHIP, CNS_ENOMEM, CNS_ENUMERIC = 0, -1, -2, -3, -4
+CNS_IMPL_TILED, CNS_IMPL_DIRECT = 0, 1
+
+_lib = None
+
+
+class CnsError(RuntimeError):
+    """A C ABI call returned a negative CNS_E* code."""
+
+    def __init__(self, code, message):
+        super().__init__(f"{message} (code {code})")
+        self.code = code
+
+
+class CnsNumericError(CnsError):
+    """The Courant number was not finite (the state is not physical)."""
+
+
+def header_symbols(path: str = HEADER):
+    """Every function name declared in include/cns.h."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cns_\w+)\s*\(", text)))
+
+
+_I3 = C.c_int * 3
+_D3 = C.c_double * 3
+_VP = C.c_void_p
+_DBL = C.c_double
+_INT = C.c_int
+
+_SIGS = {
+    "cns_last_error_string": (C.c_char_p, []),
+    "cns_set_stream": (_INT, [_VP]),
+    "cns_get_stream": (_VP, []),
+    "cns_set_stencil_impl": (_INT, [_INT]),
+    "cns_get_stencil_impl": (_INT, []),
+    "cns_fill_boundary": (_INT, [_VP, _I3, _I3]),
+    "cns_ctoprim": (_INT, [_I3, _I3, _VP, _VP, _D3, _INT, C.POINTER(C.c_double)]),
+    "cns_hypterm": (_INT, [_I3, _I3, _INT, _D3, _VP, _VP, _VP]),
+    "cns_diffterm": (_INT, [_I3, _I3, _INT, _D3, _VP, _VP, _DBL, _DBL]),
+    "cns_rk_update": (_INT, [_INT, _I3, _I3, _INT, _VP, _VP, _VP, _VP, _DBL]),
+    "cns_ctx_create": (_INT, [C.POINTER(_VP), _I3, _D3, _DBL, _DBL, _DBL]),
+    "cns_ctx_destroy": (_INT, [_VP]),
+    "cns_advance": (_INT, [_VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(C.c_double)]),
+    "cns_advance_async": (_INT, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "cns_ctx_profile": (_INT, [_VP, _INT]),
+    "cns_ctx_profile_read": (_INT, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+}
+
+PROF_KINDS = ("fill", "ctoprim", "diffterm", "hypterm", "rk", "hypterm_rk", "fill_ctoprim", "other")
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load the library once; raise if it is absent (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run `python -m cns_nospec_amd.build` "
+                          "(the HIP library has no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def register(name, restype, argtypes):
+    """Declare the ctypes signature of a further entry point."""
+    fn = getattr(load(), name)
+    fn.restype = restype
+    fn.argtypes = argtypes
+    return fn
+
+
+def check(rc: int):
+    if rc != CNS_OK:
+        msg = load().cns_last_error_string().decode(errors="replace")
+        if rc == CNS_ENUMERIC:
+            raise CnsNumericError(rc, msg)
+        raise CnsError(rc, msg)
+    return rc

@@ -1,0 +1,55 @@
+"""Builds libcns_hip.so in-tree with hipcc for gfx950.
+
+    python -m cns_nospec_amd.build [--verbose]
+
+The library is one shared object: HIP kernels plus the extern "C" entry
+points declared in include/cns.h.  Device code is built with
+-ffp-contract=off so that every kernel keeps the reference's rounding
+(bit-identical results), and with packed FP32 instructions disabled.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libcns_hip.so")
+ARCH = "gfx950"
+
+SOURCES = ["cns_kernels.hip", "cns_stencil.hip", "cns_abi.hip", "cns_batch.hip"]
+HEADERS = ["cns_common.h", "cns_kernels.h", "cns_batch.h"]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP library cannot be built")
+
+
+def build(verbose: bool = False, extra=()) -> str:
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+           "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           f"-I{INCLUDE}", f"-I{CSRC}", "-Wno-unused-result", "-o", tmp, *srcs, *extra]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    errs = "\n".join(ln for ln in (res.stdout + res.stderr).splitlines()
+                     if "packed-fp32-ops" not in ln)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{errs}")
+    if verbose and errs.strip():
+        print(errs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="--verbose" in sys.argv))

@@ -1,0 +1,39 @@
+// cns_kernels.h -- host launchers of the CNS_NoSpec HIP kernels (internal).
+#pragma once
+
+#include "cns_common.h"
+
+namespace cns {
+
+// Ghost shell, closed-form periodic image (cns_kernels.hip).
+hipError_t launch_fill_boundary(const Geom &g, double *u, hipStream_t s);
+
+// Resets a Courant accumulator slot to the bits of max(init, 0) (cns_kernels.hip).
+hipError_t launch_courno_reset(unsigned long long *acc, double init, hipStream_t s);
+
+// ctoprim over the ghosted box; acc != nullptr adds the Courant max reduction.
+hipError_t launch_ctoprim(const Geom &g, const double *u, double *q, unsigned long long *acc,
+                          double dx0, double dx1, double dx2, hipStream_t s);
+
+// dt = cfl / max(acc, 1e-50) into dt_dev (and the Courant number into courno_dev).
+hipError_t launch_courno_to_dt(const unsigned long long *acc, double cfl, double *dt_dev,
+                               double *courno_dev, hipStream_t s);
+
+// RK3 stage update; dt from dt_dev when non-null, else dt_host.
+hipError_t launch_rk_update(const Geom &g, int phase, double *u, double *unew, const double *d,
+                            const double *f, const double *dt_dev, double dt_host, hipStream_t s);
+
+// Reference-order restatements, one thread per output cell, direct global loads
+// (cns_kernels.hip).  Used by the free-standing ABI entry points' cross-check path.
+hipError_t launch_hypterm_direct(const Geom &g, const double *cons, const double *q, double *flux,
+                                 double dxi0, double dxi1, double dxi2, hipStream_t s);
+hipError_t launch_diffterm_direct(const Geom &g, const double *q, double *difflux, double dxi0,
+                                  double dxi1, double dxi2, double eta, double alam, hipStream_t s);
+
+// z-marching, LDS-tiled stencil kernels (cns_stencil.hip).
+hipError_t launch_hypterm_tiled(const Geom &g, const double *cons, const double *q, double *flux,
+                                double dxi0, double dxi1, double dxi2, hipStream_t s);
+hipError_t launch_diffterm_tiled(const Geom &g, const double *q, double *difflux, double dxi0,
+                                 double dxi1, double dxi2, double eta, double alam, hipStream_t s);
+
+}  // namespace cns
