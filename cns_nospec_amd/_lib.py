@@ -60,6 +60,7 @@ _SIGS = {
     "cns_advance": (_INT, [_VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(C.c_double)]),
     "cns_advance_async": (_INT, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "cns_ctx_profile": (_INT, [_VP, _INT]),
+    "cns_ctx_set_fused": (_INT, [_VP, _INT]),
     "cns_ctx_profile_read": (_INT, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
 }
 

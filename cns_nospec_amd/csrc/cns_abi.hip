@@ -159,6 +159,10 @@ struct cns_ctx {
     // one device block: [0] Courant accumulator bits, [1] dt, [2] courno
     unsigned long long *d_acc;
     double *d_dt, *d_courno;
+    // Stage buffer of the fused step (ghosted, 5 comps): holds U^{1/3}, so that the
+    // fused hypterm+update never writes an array it reads with halos.  Null when the
+    // unfused composition is selected.
+    double *d_stage;
     Profiler prof;
 };
 
@@ -301,7 +305,32 @@ int cns_ctx_create(cns_ctx **ctx, const int ncells[3], const double dx[3], doubl
     c->d_acc = (unsigned long long *)blk;
     c->d_dt = (double *)blk + 1;
     c->d_courno = (double *)blk + 2;
+    c->d_stage = nullptr;
+    if (hipMalloc((void **)&c->d_stage, (size_t)NC * c->g.cg * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(blk);
+        delete c;
+        return fail(CNS_ENOMEM, "hipMalloc of the %zu-byte stage buffer failed",
+                    (size_t)NC * c->g.cg * sizeof(double));
+    }
     *ctx = c;
+    return CNS_OK;
+}
+
+int cns_ctx_set_fused(cns_ctx *ctx, int fused)
+{
+    if (!ctx) return fail(CNS_EINVAL, "NULL context");
+    if (!fused && ctx->d_stage) {
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipFree(ctx->d_stage));
+        ctx->d_stage = nullptr;
+    } else if (fused && !ctx->d_stage) {
+        if (hipMalloc((void **)&ctx->d_stage, (size_t)NC * ctx->g.cg * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->d_stage = nullptr;
+            return fail(CNS_ENOMEM, "hipMalloc of the stage buffer failed");
+        }
+    }
     return CNS_OK;
 }
 
@@ -309,6 +338,7 @@ int cns_ctx_destroy(cns_ctx *ctx)
 {
     if (!ctx) return CNS_OK;
     (void)hipFree(ctx->d_acc);
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
     delete ctx;
     return CNS_OK;
 }
@@ -320,6 +350,33 @@ int cns_advance_async(cns_ctx *ctx, double *U, double *Unew, double *Q, double *
     const Geom &g = ctx->g;
     hipStream_t s = t_stream;
     double *dt = dt_dev ? dt_dev : ctx->d_dt;
+    if (ctx->d_stage) {
+        // Fused composition: each stage is fill, ctoprim, diffterm, and hypterm with the
+        // RK update of its cell applied in the same pass (F is still written).  Stage
+        // results go to a buffer the stage does not read with halos:
+        //   stage 1: U -> stage (U^{1/3});  stage 2: stage -> Unew (U^{2/3});
+        //   stage 3: Unew -> U (U^{n+1}).
+        // Every ABI array ends as the reference leaves it (cpu/advance.cpp:13-154).
+        double *T = ctx->d_stage;
+        TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, U, s));
+        HIPCHK(launch_courno_reset(ctx->d_acc, 1.0E-50, s));
+        TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, U, Q, ctx->d_acc, ctx->dx[0], ctx->dx[1], ctx->dx[2], s));
+        HIPCHK(launch_courno_to_dt(ctx->d_acc, ctx->cfl, dt, courno_dev ? courno_dev : ctx->d_courno, s));
+        TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, s));
+        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_rk(g, U, Q, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, T, dt, 1, s));
+
+        TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, T, s));
+        TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, T, Q, nullptr, 0.0, 0.0, 0.0, s));
+        TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, s));
+        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_rk(g, T, Q, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, Unew, dt, 2, s));
+
+        TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, Unew, s));
+        TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, Unew, Q, nullptr, 0.0, 0.0, 0.0, s));
+        TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, s));
+        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_rk(g, Unew, Q, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, U, dt, 3, s));
+        return CNS_OK;
+    }
+    // Unfused composition, kernel for kernel as the reference.
     // cpu/advance.cpp:45-85: stage 1
     TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, U, s));
     HIPCHK(launch_courno_reset(ctx->d_acc, 1.0E-50, s));

@@ -33,6 +33,13 @@ hipError_t launch_diffterm_direct(const Geom &g, const double *q, double *difflu
 // z-marching, LDS-tiled stencil kernels (cns_stencil.hip).
 hipError_t launch_hypterm_tiled(const Geom &g, const double *cons, const double *q, double *flux,
                                 double dxi0, double dxi1, double dxi2, hipStream_t s);
+// hypterm with the RK3 stage update of its cell fused in: flux as cns_hypterm, then
+// phase 1: out = cons + dt(d+f); 2: out = 3/4 base + 1/4(cons + dt(d+f));
+// 3: out = 1/3 base + 2/3(cons + dt(d+f)).  out must not alias cons.
+hipError_t launch_hypterm_rk(const Geom &g, const double *cons, const double *q, double *flux,
+                             double dxi0, double dxi1, double dxi2, const double *d,
+                             const double *base, double *out, const double *dt_dev, int phase,
+                             hipStream_t s);
 hipError_t launch_diffterm_tiled(const Geom &g, const double *q, double *difflux, double dxi0,
                                  double dxi1, double dxi2, double eta, double alam, hipStream_t s);
 

@@ -2,26 +2,30 @@
 //
 // Tiling (both kernels).  A workgroup owns a 64 x BY column tile of the interior
 // (x unit-stride: one 64-lane wave per row, so every global access of a row is a
-// coalesced 512-byte burst) and marches up a run of KZ planes.  Each lane keeps a
-// 9-plane register window of the fields its own column needs for the z
-// derivatives; each plane's x/y neighbourhood (4-cell halos, and for diffterm the
-// edge halos of the mixed derivatives) is staged in LDS.  Nothing is re-read
-// from HBM except the thin halos, which neighbouring workgroups fetch at the same
-// time and so come from L2.  The workgroup -> tile map groups neighbouring tiles
-// on one XCD (workgroups are dealt to the 8 XCDs round-robin), so halos are
-// shared within one L2.
+// coalesced 512-byte burst) and marches up a run of KZ planes.  Each plane's x/y
+// neighbourhood (4-cell halos, and for diffterm the edge halos of the mixed
+// derivatives) is staged in LDS.  The z neighbours of a lane's own column come
+// either from a 9-plane register window (ZWIN bit set) or straight from global
+// memory, where the planes the workgroup touched in the last few iterations are
+// still in L1/L2; the choice trades registers (occupancy) against L1/L2 traffic.
+// Nothing is re-read from HBM except the thin x/y halos, which neighbouring
+// workgroups fetch at the same time.  The workgroup -> tile map gives each XCD
+// (workgroups are dealt to the 8 XCDs round-robin) a contiguous run of tiles, so
+// halos are shared within one L2.
 //
 // Arithmetic follows the reference's per-cell operation order exactly
-// (cpu/hypterm.cpp:11-209, cpu/diffterm.cpp:22-320), so with -ffp-contract=off
-// the results are bit-identical to the reference cpu/ path.
+// (cpu/hypterm.cpp:11-209, cpu/diffterm.cpp:22-320, cpu/advance.cpp:78-153), so
+// with -ffp-contract=off the results are bit-identical to the reference cpu/ path.
+#include <cstdlib>
+
 #include "cns_kernels.h"
 
 namespace cns {
 
 namespace {
 
-constexpr int TX = 64;        // tile width (one wave)
-constexpr int PX = TX + 8;    // LDS pitch of arrays with x halos
+constexpr int TX = 64;      // tile width (one wave)
+constexpr int PX = TX + 8;  // LDS pitch of arrays with x halos
 
 // Workgroup -> tile id such that each XCD (workgroups b with equal b % 8) gets a
 // contiguous run of tile ids, i.e. spatially adjacent tiles.
@@ -56,13 +60,50 @@ __device__ __forceinline__ void corner_rc(int h, int &r, int &c)
     c = cc < 4 ? cc : TX + cc;
 }
 
+// Keeps the scheduler from hoisting the loads of later groups of a long
+// straight-line block above earlier ones (bounds the live registers).
+#define SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+
+// First and second 8th-order differences of an offset accessor F(m), before the
+// 1/dx factor (cpu/hypterm.cpp:43-47, cpu/diffterm.cpp:77-81 and :159-163).
+#define D1(F) (ALP * (F(1) - F(-1)) + BET * (F(2) - F(-2)) + GAM * (F(3) - F(-3)) + DEL * (F(4) - F(-4)))
+#define D2(F)                                                                                \
+    (CENTER * F(0) + OFF1 * (F(1) + F(-1)) + OFF2 * (F(2) + F(-2)) + OFF3 * (F(3) + F(-3)) \
+     + OFF4 * (F(4) + F(-4)))
+
+// Hypterm sums of one direction (cpu/hypterm.cpp x :49-90, y :117-147, z :174-204) for
+// momentum flux with the normal velocity UN(m), pressure P(m) and field C(m).
+#define HMOM(C, UN) \
+    (ALP * (C(1) * UN(1) - C(-1) * UN(-1)) + BET * (C(2) * UN(2) - C(-2) * UN(-2)) \
+     + GAM * (C(3) * UN(3) - C(-3) * UN(-3)) + DEL * (C(4) * UN(4) - C(-4) * UN(-4)))
+#define HMOMP(C, UN, P)                                                                  \
+    (ALP * (C(1) * UN(1) - C(-1) * UN(-1) + (P(1) - P(-1)))                              \
+     + BET * (C(2) * UN(2) - C(-2) * UN(-2) + (P(2) - P(-2)))                            \
+     + GAM * (C(3) * UN(3) - C(-3) * UN(-3) + (P(3) - P(-3)))                            \
+     + DEL * (C(4) * UN(4) - C(-4) * UN(-4) + (P(4) - P(-4))))
+#define HENE(C, UN, P)                                                                   \
+    (ALP * (C(1) * UN(1) - C(-1) * UN(-1) + (P(1) * UN(1) - P(-1) * UN(-1)))             \
+     + BET * (C(2) * UN(2) - C(-2) * UN(-2) + (P(2) * UN(2) - P(-2) * UN(-2)))           \
+     + GAM * (C(3) * UN(3) - C(-3) * UN(-3) + (P(3) * UN(3) - P(-3) * UN(-3)))           \
+     + DEL * (C(4) * UN(4) - C(-4) * UN(-4) + (P(4) * UN(4) - P(-4) * UN(-4))))
+
 // ======================================================================= hypterm
-// LDS fields of one plane: mx, my, mz, E, p (x and y halos), u (x halo), v (y halo).
-template <int BY>
-__global__ void __launch_bounds__(TX *BY)
-hypterm_tiled_kernel(Geom g, const double *__restrict__ cons, const double *__restrict__ q,
-                     double *__restrict__ flux, double dxi0, double dxi1, double dxi2, int kz,
-                     int ntx, int nty)
+// ZWIN bits: 1 mx, 2 my, 4 mz, 8 E, 16 w, 32 p -- fields whose z window is in registers.
+// With RK the kernel also applies the stage update of cpu/advance.cpp (phase 1..3)
+// for its cell, reading D and the stage base and writing `out` (never `cons`).
+struct RkArgs {
+    const double *d;     // difflux (interior)
+    const double *base;  // U^n (ghosted), phases 2 and 3
+    double *out;         // result (ghosted); must not alias cons
+    const double *dt;    // device dt
+    int phase;
+};
+
+template <int BY, int WPE, int ZWIN, bool RK>
+__global__ void __launch_bounds__(TX *BY, WPE)
+hypterm_kernel(Geom g, const double *__restrict__ cons, const double *__restrict__ q,
+               double *__restrict__ flux, double dxi0, double dxi1, double dxi2, int kz, int ntx,
+               int nty, RkArgs rk)
 {
     constexpr int NT = TX * BY, R = BY + 8, S = R * PX;
     __shared__ double sm[7 * S];
@@ -83,31 +124,43 @@ hypterm_tiled_kernel(Geom g, const double *__restrict__ cons, const double *__re
     const double *q_u = q + QU * cg, *q_v = q + QV * cg, *q_w = q + QW * cg, *q_p = q + QPRES * cg;
     const int64_t own = (int64_t)gj * g.xg + gi;
     const int me = (ty + 4) * PX + tx + 4;
+    double dt = 0.0;
+    if (RK) dt = rk.dt[0];
 
     double wmx[9], wmy[9], wmz[9], wE[9], ww[9], wp[9];
-    if (inbox) {
+#define LOADW(m, o)                                \
+    do {                                           \
+        if (ZWIN & 1) wmx[m] = c_mx[o];            \
+        if (ZWIN & 2) wmy[m] = c_my[o];            \
+        if (ZWIN & 4) wmz[m] = c_mz[o];            \
+        if (ZWIN & 8) wE[m] = c_E[o];              \
+        if (ZWIN & 16) ww[m] = q_w[o];             \
+        if (ZWIN & 32) wp[m] = q_p[o];             \
+    } while (0)
+    if (ZWIN && inbox) {
 #pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const int64_t o = (int64_t)(kbeg + m) * pg + own;
-            wmx[m] = c_mx[o]; wmy[m] = c_my[o]; wmz[m] = c_mz[o];
-            wE[m] = c_E[o]; ww[m] = q_w[o]; wp[m] = q_p[o];
-        }
+        for (int m = 0; m < 8; m++) LOADW(m, (int64_t)(kbeg + m) * pg + own);
     }
 
     for (int k = kbeg; k < kend; k++) {
         const int64_t plane = (int64_t)(k + NG) * pg;
+        const int64_t zo = plane + own;  // own cell of the centre plane
+        double c_own[6] = {0, 0, 0, 0, 0, 0};  // mx my mz E p at the centre, u, v
         double u_own = 0.0, v_own = 0.0;
         if (inbox) {
-            const int64_t o = (int64_t)(k + 2 * NG) * pg + own;
-            wmx[8] = c_mx[o]; wmy[8] = c_my[o]; wmz[8] = c_mz[o];
-            wE[8] = c_E[o]; ww[8] = q_w[o]; wp[8] = q_p[o];
-            u_own = q_u[plane + own];
-            v_own = q_v[plane + own];
+            if (ZWIN) LOADW(8, zo + 4 * pg);
+            c_own[0] = (ZWIN & 1) ? wmx[4] : c_mx[zo];
+            c_own[1] = (ZWIN & 2) ? wmy[4] : c_my[zo];
+            c_own[2] = (ZWIN & 4) ? wmz[4] : c_mz[zo];
+            c_own[3] = (ZWIN & 8) ? wE[4] : c_E[zo];
+            c_own[4] = (ZWIN & 32) ? wp[4] : q_p[zo];
+            u_own = q_u[zo];
+            v_own = q_v[zo];
         }
         __syncthreads();
         if (inbox) {
-            s_mx[me] = wmx[4]; s_my[me] = wmy[4]; s_mz[me] = wmz[4];
-            s_E[me] = wE[4]; s_p[me] = wp[4]; s_u[me] = u_own; s_v[me] = v_own;
+            s_mx[me] = c_own[0]; s_my[me] = c_own[1]; s_mz[me] = c_own[2];
+            s_E[me] = c_own[3]; s_p[me] = c_own[4]; s_u[me] = u_own; s_v[me] = v_own;
         }
         for (int h = threadIdx.x; h < 8 * BY; h += NT) {
             int r, c;
@@ -136,97 +189,77 @@ hypterm_tiled_kernel(Geom g, const double *__restrict__ cons, const double *__re
         if (active) {
             double F[5];
             // ---- x (cpu/hypterm.cpp:33-90): F = -(sum)*dxinv(1)
-            {
-#define XC(a, m) a[me + (m)]
-                const double u1 = XC(s_u, 1), u2 = XC(s_u, 2), u3 = XC(s_u, 3), u4 = XC(s_u, 4);
-                const double um1 = XC(s_u, -1), um2 = XC(s_u, -2), um3 = XC(s_u, -3), um4 = XC(s_u, -4);
-                F[IRHO] = -(ALP * (XC(s_mx, 1) - XC(s_mx, -1)) + BET * (XC(s_mx, 2) - XC(s_mx, -2))
-                            + GAM * (XC(s_mx, 3) - XC(s_mx, -3)) + DEL * (XC(s_mx, 4) - XC(s_mx, -4))) * dxi0;
-                F[IMX] = -(ALP * (XC(s_mx, 1) * u1 - XC(s_mx, -1) * um1 + (XC(s_p, 1) - XC(s_p, -1)))
-                           + BET * (XC(s_mx, 2) * u2 - XC(s_mx, -2) * um2 + (XC(s_p, 2) - XC(s_p, -2)))
-                           + GAM * (XC(s_mx, 3) * u3 - XC(s_mx, -3) * um3 + (XC(s_p, 3) - XC(s_p, -3)))
-                           + DEL * (XC(s_mx, 4) * u4 - XC(s_mx, -4) * um4 + (XC(s_p, 4) - XC(s_p, -4)))) * dxi0;
-                F[IMY] = -(ALP * (XC(s_my, 1) * u1 - XC(s_my, -1) * um1)
-                           + BET * (XC(s_my, 2) * u2 - XC(s_my, -2) * um2)
-                           + GAM * (XC(s_my, 3) * u3 - XC(s_my, -3) * um3)
-                           + DEL * (XC(s_my, 4) * u4 - XC(s_my, -4) * um4)) * dxi0;
-                F[IMZ] = -(ALP * (XC(s_mz, 1) * u1 - XC(s_mz, -1) * um1)
-                           + BET * (XC(s_mz, 2) * u2 - XC(s_mz, -2) * um2)
-                           + GAM * (XC(s_mz, 3) * u3 - XC(s_mz, -3) * um3)
-                           + DEL * (XC(s_mz, 4) * u4 - XC(s_mz, -4) * um4)) * dxi0;
-                F[IENE] = -(ALP * (XC(s_E, 1) * u1 - XC(s_E, -1) * um1 + (XC(s_p, 1) * u1 - XC(s_p, -1) * um1))
-                            + BET * (XC(s_E, 2) * u2 - XC(s_E, -2) * um2 + (XC(s_p, 2) * u2 - XC(s_p, -2) * um2))
-                            + GAM * (XC(s_E, 3) * u3 - XC(s_E, -3) * um3 + (XC(s_p, 3) * u3 - XC(s_p, -3) * um3))
-                            + DEL * (XC(s_E, 4) * u4 - XC(s_E, -4) * um4 + (XC(s_p, 4) * u4 - XC(s_p, -4) * um4))) * dxi0;
-#undef XC
-            }
+#define XMX(m) s_mx[me + (m)]
+#define XMY(m) s_my[me + (m)]
+#define XMZ(m) s_mz[me + (m)]
+#define XE(m) s_E[me + (m)]
+#define XP(m) s_p[me + (m)]
+#define XU(m) s_u[me + (m)]
+            F[IRHO] = -(D1(XMX)) * dxi0;
+            F[IMX] = -(HMOMP(XMX, XU, XP)) * dxi0;
+            F[IMY] = -(HMOM(XMY, XU)) * dxi0;
+            F[IMZ] = -(HMOM(XMZ, XU)) * dxi0;
+            F[IENE] = -(HENE(XE, XU, XP)) * dxi0;
             // ---- y (cpu/hypterm.cpp:101-147): F = F - (sum)*dxinv(2)
-            {
-#define YC(a, m) a[me + (m) * PX]
-                const double v1 = YC(s_v, 1), v2 = YC(s_v, 2), v3 = YC(s_v, 3), v4 = YC(s_v, 4);
-                const double vm1 = YC(s_v, -1), vm2 = YC(s_v, -2), vm3 = YC(s_v, -3), vm4 = YC(s_v, -4);
-                F[IRHO] = F[IRHO] - (ALP * (YC(s_my, 1) - YC(s_my, -1)) + BET * (YC(s_my, 2) - YC(s_my, -2))
-                                     + GAM * (YC(s_my, 3) - YC(s_my, -3)) + DEL * (YC(s_my, 4) - YC(s_my, -4))) * dxi1;
-                F[IMX] = F[IMX] - (ALP * (YC(s_mx, 1) * v1 - YC(s_mx, -1) * vm1)
-                                   + BET * (YC(s_mx, 2) * v2 - YC(s_mx, -2) * vm2)
-                                   + GAM * (YC(s_mx, 3) * v3 - YC(s_mx, -3) * vm3)
-                                   + DEL * (YC(s_mx, 4) * v4 - YC(s_mx, -4) * vm4)) * dxi1;
-                F[IMY] = F[IMY] - (ALP * (YC(s_my, 1) * v1 - YC(s_my, -1) * vm1 + (YC(s_p, 1) - YC(s_p, -1)))
-                                   + BET * (YC(s_my, 2) * v2 - YC(s_my, -2) * vm2 + (YC(s_p, 2) - YC(s_p, -2)))
-                                   + GAM * (YC(s_my, 3) * v3 - YC(s_my, -3) * vm3 + (YC(s_p, 3) - YC(s_p, -3)))
-                                   + DEL * (YC(s_my, 4) * v4 - YC(s_my, -4) * vm4 + (YC(s_p, 4) - YC(s_p, -4)))) * dxi1;
-                F[IMZ] = F[IMZ] - (ALP * (YC(s_mz, 1) * v1 - YC(s_mz, -1) * vm1)
-                                   + BET * (YC(s_mz, 2) * v2 - YC(s_mz, -2) * vm2)
-                                   + GAM * (YC(s_mz, 3) * v3 - YC(s_mz, -3) * vm3)
-                                   + DEL * (YC(s_mz, 4) * v4 - YC(s_mz, -4) * vm4)) * dxi1;
-                F[IENE] = F[IENE] - (ALP * (YC(s_E, 1) * v1 - YC(s_E, -1) * vm1 + (YC(s_p, 1) * v1 - YC(s_p, -1) * vm1))
-                                     + BET * (YC(s_E, 2) * v2 - YC(s_E, -2) * vm2 + (YC(s_p, 2) * v2 - YC(s_p, -2) * vm2))
-                                     + GAM * (YC(s_E, 3) * v3 - YC(s_E, -3) * vm3 + (YC(s_p, 3) * v3 - YC(s_p, -3) * vm3))
-                                     + DEL * (YC(s_E, 4) * v4 - YC(s_E, -4) * vm4 + (YC(s_p, 4) * v4 - YC(s_p, -4) * vm4))) * dxi1;
-#undef YC
-            }
-            // ---- z (cpu/hypterm.cpp:158-204) from the register window, plane k+m at [4+m]
-            {
-#define ZC(a, m) a[4 + (m)]
-                F[IRHO] = F[IRHO] - (ALP * (ZC(wmz, 1) - ZC(wmz, -1)) + BET * (ZC(wmz, 2) - ZC(wmz, -2))
-                                     + GAM * (ZC(wmz, 3) - ZC(wmz, -3)) + DEL * (ZC(wmz, 4) - ZC(wmz, -4))) * dxi2;
-                F[IMX] = F[IMX] - (ALP * (ZC(wmx, 1) * ZC(ww, 1) - ZC(wmx, -1) * ZC(ww, -1))
-                                   + BET * (ZC(wmx, 2) * ZC(ww, 2) - ZC(wmx, -2) * ZC(ww, -2))
-                                   + GAM * (ZC(wmx, 3) * ZC(ww, 3) - ZC(wmx, -3) * ZC(ww, -3))
-                                   + DEL * (ZC(wmx, 4) * ZC(ww, 4) - ZC(wmx, -4) * ZC(ww, -4))) * dxi2;
-                F[IMY] = F[IMY] - (ALP * (ZC(wmy, 1) * ZC(ww, 1) - ZC(wmy, -1) * ZC(ww, -1))
-                                   + BET * (ZC(wmy, 2) * ZC(ww, 2) - ZC(wmy, -2) * ZC(ww, -2))
-                                   + GAM * (ZC(wmy, 3) * ZC(ww, 3) - ZC(wmy, -3) * ZC(ww, -3))
-                                   + DEL * (ZC(wmy, 4) * ZC(ww, 4) - ZC(wmy, -4) * ZC(ww, -4))) * dxi2;
-                F[IMZ] = F[IMZ] - (ALP * (ZC(wmz, 1) * ZC(ww, 1) - ZC(wmz, -1) * ZC(ww, -1) + (ZC(wp, 1) - ZC(wp, -1)))
-                                   + BET * (ZC(wmz, 2) * ZC(ww, 2) - ZC(wmz, -2) * ZC(ww, -2) + (ZC(wp, 2) - ZC(wp, -2)))
-                                   + GAM * (ZC(wmz, 3) * ZC(ww, 3) - ZC(wmz, -3) * ZC(ww, -3) + (ZC(wp, 3) - ZC(wp, -3)))
-                                   + DEL * (ZC(wmz, 4) * ZC(ww, 4) - ZC(wmz, -4) * ZC(ww, -4) + (ZC(wp, 4) - ZC(wp, -4)))) * dxi2;
-                F[IENE] = F[IENE] - (ALP * (ZC(wE, 1) * ZC(ww, 1) - ZC(wE, -1) * ZC(ww, -1) + (ZC(wp, 1) * ZC(ww, 1) - ZC(wp, -1) * ZC(ww, -1)))
-                                     + BET * (ZC(wE, 2) * ZC(ww, 2) - ZC(wE, -2) * ZC(ww, -2) + (ZC(wp, 2) * ZC(ww, 2) - ZC(wp, -2) * ZC(ww, -2)))
-                                     + GAM * (ZC(wE, 3) * ZC(ww, 3) - ZC(wE, -3) * ZC(ww, -3) + (ZC(wp, 3) * ZC(ww, 3) - ZC(wp, -3) * ZC(ww, -3)))
-                                     + DEL * (ZC(wE, 4) * ZC(ww, 4) - ZC(wE, -4) * ZC(ww, -4) + (ZC(wp, 4) * ZC(ww, 4) - ZC(wp, -4) * ZC(ww, -4)))) * dxi2;
-#undef ZC
-            }
+#define YMX(m) s_mx[me + (m) * PX]
+#define YMY(m) s_my[me + (m) * PX]
+#define YMZ(m) s_mz[me + (m) * PX]
+#define YE(m) s_E[me + (m) * PX]
+#define YP(m) s_p[me + (m) * PX]
+#define YV(m) s_v[me + (m) * PX]
+            F[IRHO] = F[IRHO] - (D1(YMY)) * dxi1;
+            F[IMX] = F[IMX] - (HMOM(YMX, YV)) * dxi1;
+            F[IMY] = F[IMY] - (HMOMP(YMY, YV, YP)) * dxi1;
+            F[IMZ] = F[IMZ] - (HMOM(YMZ, YV)) * dxi1;
+            F[IENE] = F[IENE] - (HENE(YE, YV, YP)) * dxi1;
+            // ---- z (cpu/hypterm.cpp:158-204) from the window or global memory
+#define ZMX(m) ((ZWIN & 1) ? wmx[4 + (m)] : c_mx[zo + (int64_t)(m) * pg])
+#define ZMY(m) ((ZWIN & 2) ? wmy[4 + (m)] : c_my[zo + (int64_t)(m) * pg])
+#define ZMZ(m) ((ZWIN & 4) ? wmz[4 + (m)] : c_mz[zo + (int64_t)(m) * pg])
+#define ZE(m) ((ZWIN & 8) ? wE[4 + (m)] : c_E[zo + (int64_t)(m) * pg])
+#define ZW(m) ((ZWIN & 16) ? ww[4 + (m)] : q_w[zo + (int64_t)(m) * pg])
+#define ZP(m) ((ZWIN & 32) ? wp[4 + (m)] : q_p[zo + (int64_t)(m) * pg])
+            F[IRHO] = F[IRHO] - (D1(ZMZ)) * dxi2;
+            F[IMX] = F[IMX] - (HMOM(ZMX, ZW)) * dxi2;
+            F[IMY] = F[IMY] - (HMOM(ZMY, ZW)) * dxi2;
+            F[IMZ] = F[IMZ] - (HMOMP(ZMZ, ZW, ZP)) * dxi2;
+            F[IENE] = F[IENE] - (HENE(ZE, ZW, ZP)) * dxi2;
             const int64_t fo = iidx(g, gi - NG, gj - NG, k);
 #pragma unroll
             for (int l = 0; l < NC; l++) flux[l * g.c + fo] = F[l];
-        }
+            if (RK) {
+                // cpu/advance.cpp:82 (phase 1), :114-116 (phase 2), :148-150 (phase 3)
+                const double uc[5] = {cons[zo], c_own[0], c_own[1], c_own[2], c_own[3]};
 #pragma unroll
-        for (int m = 0; m < 8; m++) {
-            wmx[m] = wmx[m + 1]; wmy[m] = wmy[m + 1]; wmz[m] = wmz[m + 1];
-            wE[m] = wE[m + 1]; ww[m] = ww[m + 1]; wp[m] = wp[m + 1];
+                for (int l = 0; l < NC; l++) {
+                    const double df = rk.d[l * g.c + fo] + F[l];
+                    double r;
+                    if (rk.phase == 1)
+                        r = uc[l] + dt * df;
+                    else if (rk.phase == 2)
+                        r = ThreeQuarters * rk.base[l * cg + zo] + OneQuarter * (uc[l] + dt * df);
+                    else
+                        r = OneThird * rk.base[l * cg + zo] + TwoThirds * (uc[l] + dt * df);
+                    rk.out[l * cg + zo] = r;
+                }
+            }
+        }
+        if (ZWIN) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                if (ZWIN & 1) wmx[m] = wmx[m + 1];
+                if (ZWIN & 2) wmy[m] = wmy[m + 1];
+                if (ZWIN & 4) wmz[m] = wmz[m + 1];
+                if (ZWIN & 8) wE[m] = wE[m + 1];
+                if (ZWIN & 16) ww[m] = ww[m + 1];
+                if (ZWIN & 32) wp[m] = wp[m + 1];
+            }
         }
     }
+#undef LOADW
 }
 
 // ====================================================================== diffterm
-// First and second 8th-order differences of a value accessor.
-#define D1(F) (ALP * (F(1) - F(-1)) + BET * (F(2) - F(-2)) + GAM * (F(3) - F(-3)) + DEL * (F(4) - F(-4)))
-#define D2(F)                                                                                \
-    (CENTER * F(0) + OFF1 * (F(1) + F(-1)) + OFF2 * (F(2) + F(-2)) + OFF3 * (F(3) + F(-3)) \
-     + OFF4 * (F(4) + F(-4)))
-
 // Per plane k (ghost-shifted kc = k+4) three LDS phases:
 //  A  stage the leading plane k+4 (u with x halo, v with y halo) and the centre plane
 //     k (u, v with square halos, w and T with cross halos);
@@ -236,6 +269,8 @@ hypterm_tiled_kernel(Geom g, const double *__restrict__ cons, const double *__re
 //  C  every output of cpu/diffterm.cpp:155-315 for the lane's own cell.
 // The reference's nine ghosted temporaries are never materialised; each value it
 // would store is computed here by the same expression.
+// ZWIN bits: 1 u, 2 v, 4 w, 8 T -- fields whose z window is in registers; ux and vy
+// (computed values) always have register windows.
 template <int BY>
 struct DiffLds {
     static constexpr int R = BY + 8;
@@ -251,10 +286,10 @@ struct DiffLds {
     static constexpr int TOTAL = DWZ + R * PX;
 };
 
-template <int BY>
-__global__ void __launch_bounds__(TX *BY)
-diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__ difflux, double dxi0,
-                      double dxi1, double dxi2, double eta, double alam, int kz, int ntx, int nty)
+template <int BY, int WPE, int ZWIN>
+__global__ void __launch_bounds__(TX *BY, WPE)
+diffterm_kernel(Geom g, const double *__restrict__ q, double *__restrict__ difflux, double dxi0,
+                double dxi1, double dxi2, double eta, double alam, int kz, int ntx, int nty)
 {
     using L = DiffLds<BY>;
     constexpr int NT = TX * BY;
@@ -281,14 +316,22 @@ diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__
 
     // windows: plane k+m at [4+m]
     double wu[9], wv[9], ww[9], wt[9], wux[9], wvy[9];
-    if (inbox) {
+    if (ZWIN && inbox) {
 #pragma unroll
         for (int m = 0; m < 8; m++) {
             const int64_t o = (int64_t)(kbeg + m) * pg + own;
-            wu[m] = q_u[o]; wv[m] = q_v[o]; ww[m] = q_w[o]; wt[m] = q_t[o];
+            if (ZWIN & 1) wu[m] = q_u[o];
+            if (ZWIN & 2) wv[m] = q_v[o];
+            if (ZWIN & 4) ww[m] = q_w[o];
+            if (ZWIN & 8) wt[m] = q_t[o];
         }
     }
-    // ux, vy of the first 8 planes of the window: staged through LDS one plane at a time
+#define LUX(mm) s_lu[meby + (mm)]
+#define LVY(mm) s_lv[me64 + (mm) * TX]
+    // ux, vy of the first 8 planes of the window: staged through LDS one plane at a
+    // time (a rolled loop: unrolled, its barriers make the compiler keep every
+    // plane's loads live at once)
+#pragma unroll 1
     for (int m = 0; m < 8; m++) {
         const int64_t plane = (int64_t)(kbeg + m) * pg;
         __syncthreads();
@@ -309,28 +352,43 @@ diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__
             if (hi < g.g0 && hj < g.g1) s_lv[r * TX + c - 4] = q_v[plane + (int64_t)hj * xg + hi];
         }
         __syncthreads();
-#define LUX(mm) s_lu[meby + (mm)]
-#define LVY(mm) s_lv[me64 + (mm) * TX]
-        const double uxm = D1(LUX) * dxi0;
-        const double vym = D1(LVY) * dxi1;
+        {
+            // shift in from the top so that after 8 planes slot m holds plane kbeg+m
+            const double ux_ = D1(LUX) * dxi0, vy_ = D1(LVY) * dxi1;
 #pragma unroll
-        for (int mm = 0; mm < 8; mm++)
-            if (mm == m) { wux[mm] = uxm; wvy[mm] = vym; }
+            for (int mm = 0; mm < 7; mm++) {
+                wux[mm] = wux[mm + 1];
+                wvy[mm] = wvy[mm + 1];
+            }
+            wux[7] = ux_;
+            wvy[7] = vy_;
+        }
     }
 
     for (int k = kbeg; k < kend; k++) {
         const int64_t cplane = (int64_t)(k + NG) * pg;
         const int64_t lplane = (int64_t)(k + 2 * NG) * pg;
+        const int64_t zo = cplane + own;
+        double lead_u = 0.0, lead_v = 0.0, cen[4] = {0, 0, 0, 0};
         if (inbox) {
             const int64_t o = lplane + own;
-            wu[8] = q_u[o]; wv[8] = q_v[o]; ww[8] = q_w[o]; wt[8] = q_t[o];
+            lead_u = q_u[o];
+            lead_v = q_v[o];
+            if (ZWIN & 1) wu[8] = lead_u;
+            if (ZWIN & 2) wv[8] = lead_v;
+            if (ZWIN & 4) ww[8] = q_w[o];
+            if (ZWIN & 8) wt[8] = q_t[o];
+            cen[0] = (ZWIN & 1) ? wu[4] : q_u[zo];
+            cen[1] = (ZWIN & 2) ? wv[4] : q_v[zo];
+            cen[2] = (ZWIN & 4) ? ww[4] : q_w[zo];
+            cen[3] = (ZWIN & 8) ? wt[4] : q_t[zo];
         }
         // ---------------- phase A
         __syncthreads();
         if (inbox) {
-            s_lu[meby] = wu[8];
-            s_lv[me64] = wv[8];
-            s_cu[me] = wu[4]; s_cv[me] = wv[4]; s_cw[me] = ww[4]; s_ct[me] = wt[4];
+            s_lu[meby] = lead_u;
+            s_lv[me64] = lead_v;
+            s_cu[me] = cen[0]; s_cv[me] = cen[1]; s_cw[me] = cen[2]; s_ct[me] = cen[3];
         }
         for (int h = threadIdx.x; h < 8 * BY; h += NT) {
             int r, c;
@@ -373,14 +431,11 @@ diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__
             // leading plane: ux, vy of the own column (cpu/diffterm.cpp:77-81, :110-114)
             wux[8] = D1(LUX) * dxi0;
             wvy[8] = D1(LVY) * dxi1;
-#undef LUX
-#undef LVY
-            // centre plane own values
             s_ux[me64] = wux[4];
             s_vy[meby] = wvy[4];
-#define WW(mm) ww[4 + (mm)]
-            s_wz[me] = D1(WW) * dxi2;  // cpu/diffterm.cpp:143-147
-#undef WW
+            // own wz (cpu/diffterm.cpp:143-147)
+#define ZWW(mm) ((ZWIN & 4) ? ww[4 + (mm)] : q_w[zo + (int64_t)(mm) * pg])
+            s_wz[me] = D1(ZWW) * dxi2;
             // ux on the y-halo rows (x derivative of the square u tile)
             for (int h = threadIdx.x; h < 8 * TX; h += NT) {
                 int r, c;
@@ -419,25 +474,24 @@ diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__
         if (active) {
 #define XS(a, mm) a[me + (mm)]
 #define YS(a, mm) a[me + (mm) * PX]
-#define ZW(a, mm) a[4 + (mm)]
 #define CUx(mm) XS(s_cu, mm)
 #define CUy(mm) YS(s_cu, mm)
-#define CUz(mm) ZW(wu, mm)
+#define CUz(mm) ((ZWIN & 1) ? wu[4 + (mm)] : q_u[zo + (int64_t)(mm) * pg])
 #define CVx(mm) XS(s_cv, mm)
 #define CVy(mm) YS(s_cv, mm)
-#define CVz(mm) ZW(wv, mm)
+#define CVz(mm) ((ZWIN & 2) ? wv[4 + (mm)] : q_v[zo + (int64_t)(mm) * pg])
 #define CWx(mm) XS(s_cw, mm)
 #define CWy(mm) YS(s_cw, mm)
-#define CWz(mm) ZW(ww, mm)
+#define CWz(mm) ZWW(mm)
 #define CTx(mm) XS(s_ct, mm)
 #define CTy(mm) YS(s_ct, mm)
-#define CTz(mm) ZW(wt, mm)
+#define CTz(mm) ((ZWIN & 8) ? wt[4 + (mm)] : q_t[zo + (int64_t)(mm) * pg])
 #define VYx(mm) s_vy[meby + (mm)]
 #define WZx(mm) XS(s_wz, mm)
 #define UXy(mm) s_ux[me64 + (mm) * TX]
 #define WZy(mm) YS(s_wz, mm)
-#define UXz(mm) ZW(wux, mm)
-#define VYz(mm) ZW(wvy, mm)
+#define UXz(mm) wux[4 + (mm)]
+#define VYz(mm) wvy[4 + (mm)]
             // cpu/diffterm.cpp:159-187
             const double uxx = D2(CUx) * dxi0s;
             const double uyy = D2(CUy) * dxi1s;
@@ -445,6 +499,7 @@ diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__
             const double vyx = D1(VYx) * dxi0;
             const double wzx = D1(WZx) * dxi0;
             const double dmx = eta * (FourThirds * uxx + uyy + uzz + OneThird * (vyx + wzx));
+            SCHED_FENCE();
             // :198-226
             const double vxx = D2(CVx) * dxi0s;
             const double vyy = D2(CVy) * dxi1s;
@@ -452,6 +507,7 @@ diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__
             const double uxy = D1(UXy) * dxi1;
             const double wzy = D1(WZy) * dxi1;
             const double dmy = eta * (vxx + FourThirds * vyy + vzz + OneThird * (uxy + wzy));
+            SCHED_FENCE();
             // :237-265
             const double wxx = D2(CWx) * dxi0s;
             const double wyy = D2(CWy) * dxi1s;
@@ -459,10 +515,12 @@ diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__
             const double uxz = D1(UXz) * dxi2;
             const double vyz = D1(VYz) * dxi2;
             const double dmz = eta * (wxx + wyy + FourThirds * wzz + OneThird * (uxz + vyz));
+            SCHED_FENCE();
             // :276-311
             const double txx = D2(CTx) * dxi0s;
             const double tyy = D2(CTy) * dxi1s;
             const double tzz = D2(CTz) * dxi2s;
+            SCHED_FENCE();
             const double ux = wux[4], vy = wvy[4], wz = s_wz[me];
             const double uy = D1(CUy) * dxi1, uz = D1(CUz) * dxi2;
             const double vx = D1(CVx) * dxi0, vz = D1(CVz) * dxi2;
@@ -476,7 +534,7 @@ diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__
             const double tauyz = vz + wy;
             double mechwork = tauxx * ux + tauyy * vy + tauzz * wz + (tauxy * tauxy) + (tauxz * tauxz)
                               + (tauyz * tauyz);
-            mechwork = eta * mechwork + dmx * wu[4] + dmy * wv[4] + dmz * ww[4];
+            mechwork = eta * mechwork + dmx * cen[0] + dmy * cen[1] + dmz * cen[2];
             const int64_t fo = iidx(g, gi - NG, gj - NG, k);
             const int64_t c = g.c;
             difflux[fo] = 0.0E0;
@@ -486,7 +544,6 @@ diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__
             difflux[fo + IENE * c] = alam * (txx + tyy + tzz) + mechwork;
 #undef XS
 #undef YS
-#undef ZW
 #undef CUx
 #undef CUy
 #undef CUz
@@ -508,16 +565,18 @@ diffterm_tiled_kernel(Geom g, const double *__restrict__ q, double *__restrict__
         }
 #pragma unroll
         for (int m = 0; m < 8; m++) {
-            wu[m] = wu[m + 1]; wv[m] = wv[m + 1]; ww[m] = ww[m + 1]; wt[m] = wt[m + 1];
-            wux[m] = wux[m + 1]; wvy[m] = wvy[m + 1];
+            if (ZWIN & 1) wu[m] = wu[m + 1];
+            if (ZWIN & 2) wv[m] = wv[m + 1];
+            if (ZWIN & 4) ww[m] = ww[m + 1];
+            if (ZWIN & 8) wt[m] = wt[m + 1];
+            wux[m] = wux[m + 1];
+            wvy[m] = wvy[m + 1];
         }
     }
+#undef ZWW
+#undef LUX
+#undef LVY
 }
-#undef D1
-#undef D2
-
-constexpr int kHypBY = 4;
-constexpr int kDiffBY = 4;
 
 int pick_kz(const Geom &g, int tiles_xy)
 {
@@ -528,30 +587,80 @@ int pick_kz(const Geom &g, int tiles_xy)
     return kz;
 }
 
+template <int BY, int WPE, int ZWIN, bool RK>
+hipError_t run_hyp(const Geom &g, const double *cons, const double *q, double *flux, double dxi0,
+                   double dxi1, double dxi2, const RkArgs &rk, hipStream_t s)
+{
+    const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
+    const int kz = pick_kz(g, ntx * nty);
+    const int ntz = (g.n2 + kz - 1) / kz;
+    hypterm_kernel<BY, WPE, ZWIN, RK><<<ntx * nty * ntz, TX * BY, 0, s>>>(
+        g, cons, q, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk);
+    return hipGetLastError();
+}
+
+template <int BY, int WPE, int ZWIN>
+hipError_t run_diff(const Geom &g, const double *q, double *difflux, double dxi0, double dxi1,
+                    double dxi2, double eta, double alam, hipStream_t s)
+{
+    const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
+    const int kz = pick_kz(g, ntx * nty);
+    const int ntz = (g.n2 + kz - 1) / kz;
+    diffterm_kernel<BY, WPE, ZWIN><<<ntx * nty * ntz, TX * BY, 0, s>>>(
+        g, q, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty);
+    return hipGetLastError();
+}
+
+int env_variant(const char *name)
+{
+    const char *v = getenv(name);
+    return v ? atoi(v) : 0;
+}
+
+template <bool RK>
+hipError_t hyp_dispatch(const Geom &g, const double *cons, const double *q, double *flux,
+                        double dxi0, double dxi1, double dxi2, const RkArgs &rk, hipStream_t s)
+{
+    switch (env_variant("CNS_HYP_VARIANT")) {
+    case 1: return run_hyp<4, 2, 63, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    case 2: return run_hyp<4, 3, 48, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    case 3: return run_hyp<4, 4, 0, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    case 4: return run_hyp<8, 2, 48, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    case 5: return run_hyp<8, 2, 0, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    case 6: return run_hyp<4, 3, 0, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    default: return run_hyp<4, 1, 63, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_hypterm_tiled(const Geom &g, const double *cons, const double *q, double *flux,
                                 double dxi0, double dxi1, double dxi2, hipStream_t s)
 {
-    constexpr int BY = kHypBY;
-    const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
-    const int kz = pick_kz(g, ntx * nty);
-    const int ntz = (g.n2 + kz - 1) / kz;
-    hypterm_tiled_kernel<BY><<<ntx * nty * ntz, TX * BY, 0, s>>>(g, cons, q, flux, dxi0, dxi1, dxi2,
-                                                                kz, ntx, nty);
-    return hipGetLastError();
+    RkArgs rk{nullptr, nullptr, nullptr, nullptr, 0};
+    return hyp_dispatch<false>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+}
+
+hipError_t launch_hypterm_rk(const Geom &g, const double *cons, const double *q, double *flux,
+                             double dxi0, double dxi1, double dxi2, const double *d,
+                             const double *base, double *out, const double *dt_dev, int phase,
+                             hipStream_t s)
+{
+    RkArgs rk{d, base, out, dt_dev, phase};
+    return hyp_dispatch<true>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
 }
 
 hipError_t launch_diffterm_tiled(const Geom &g, const double *q, double *difflux, double dxi0,
                                  double dxi1, double dxi2, double eta, double alam, hipStream_t s)
 {
-    constexpr int BY = kDiffBY;
-    const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
-    const int kz = pick_kz(g, ntx * nty);
-    const int ntz = (g.n2 + kz - 1) / kz;
-    diffterm_tiled_kernel<BY><<<ntx * nty * ntz, TX * BY, 0, s>>>(g, q, difflux, dxi0, dxi1, dxi2,
-                                                                 eta, alam, kz, ntx, nty);
-    return hipGetLastError();
+    switch (env_variant("CNS_DIFF_VARIANT")) {
+    case 1: return run_diff<4, 2, 7>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
+    case 2: return run_diff<4, 2, 0>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
+    case 3: return run_diff<4, 3, 0>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
+    case 4: return run_diff<8, 2, 0>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
+    case 5: return run_diff<4, 2, 15>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
+    default: return run_diff<4, 1, 15>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
+    }
 }
 
 }  // namespace cns

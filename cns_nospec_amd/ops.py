@@ -164,6 +164,11 @@ class Stepper:
 
     advance = new_advance
 
+    def set_fused(self, fused: bool):
+        """Fused stage (hypterm + update in one pass, default) or the reference's
+        kernel-for-kernel composition; results are bit-identical."""
+        _lib.check(_lib.load().cns_ctx_set_fused(self._h, int(fused)))
+
     def profile(self, enable: bool = True):
         """Turn live per-kernel timing on/off (clears the totals)."""
         _lib.check(_lib.load().cns_ctx_profile(self._h, int(enable)))

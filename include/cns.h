@@ -95,14 +95,23 @@ int cns_rk_update(int phase, const int lo[3], const int hi[3], int ng,
 
 /* Per-box configuration and device scratch: the reference's global h_const.
  * ncells = interior extents, dx = cell sizes, cfl/eta/alam as in the reference's
- * general_input.  Allocates a few bytes of device memory; no allocation happens
- * on the step path afterwards.
+ * general_input.  Allocates a small device block and the stage buffer of the
+ * fused step (see cns_ctx_set_fused); no allocation happens on the step path.
  * Replaces: read_configurations + h_const  cpu/helper_functions.cpp:38-83, cpu/main.cpp:4;
  *           copy_configurations + d_const  gpu/util.cu:129-147, gpu/main.cu:6-9. */
 typedef struct cns_ctx cns_ctx;
 int cns_ctx_create(cns_ctx **ctx, const int ncells[3], const double dx[3],
                    double cfl, double eta, double alam);
 int cns_ctx_destroy(cns_ctx *ctx);
+
+/* Selects how cns_advance composes a stage.  fused = 1 (the default set by
+ * cns_ctx_create): fill, ctoprim, diffterm, then hypterm with the RK update of each
+ * cell applied in the same pass; the context then owns one ghosted 5-component
+ * stage buffer (the U^{1/3} state), allocated here or at creation.  fused = 0:
+ * kernel for kernel as the reference (separate hypterm and update launches), and
+ * the stage buffer is released.  Both give bit-identical results.  No
+ * counterpart in the reference. */
+int cns_ctx_set_fused(cns_ctx *ctx, int fused);
 
 /* One full RK3 time step: fill_boundary, ctoprim (+courno), dt = cfl/courno,
  * diffterm, hypterm, update, three times, dt fixed after stage 1.  On return U
