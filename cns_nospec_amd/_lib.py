@@ -11,9 +11,10 @@ import re
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libcns_hip.so")
-HEADER = os.path.join(os.path.dirname(PKG), "include", "cns.h")
+INCLUDE = os.path.join(os.path.dirname(PKG), "include")
+HEADERS = [os.path.join(INCLUDE, h) for h in ("cns.h", "cns_batch.h")]
 
-CNS_OK, CNS_EINVAL, CNS_EHIP, CNS_ENOMEM, CNS_ENUMERIC = 0, -1, -2, -3, -4
+CNS_OK, CNS_EINVAL, CNS_EHIP, CNS_ENOMEM, CNS_ENUMERIC, CNS_EBUDGET = 0, -1, -2, -3, -4, -5
 CNS_IMPL_TILED, CNS_IMPL_DIRECT = 0, 1
 
 _lib = None
@@ -31,11 +32,18 @@ class CnsNumericError(CnsError):
     """The Courant number was not finite (the state is not physical)."""
 
 
-def header_symbols(path: str = HEADER):
-    """Every function name declared in include/cns.h."""
-    text = open(path).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(cns_\w+)\s*\(", text)))
+class CnsBudgetError(CnsError):
+    """A box does not fit in the batched engine's HBM budget."""
+
+
+def header_symbols(paths=None):
+    """Every function name declared in the public headers (include/*.h)."""
+    names = set()
+    for path in paths or [os.path.join(INCLUDE, f) for f in sorted(os.listdir(INCLUDE))
+                          if f.endswith(".h")]:
+        text = re.sub(r"/\*.*?\*/", "", open(path).read(), flags=re.S)
+        names |= set(re.findall(r"\b(cns_\w+)\s*\(", text))
+    return sorted(names)
 
 
 _I3 = C.c_int * 3
@@ -62,6 +70,18 @@ _SIGS = {
     "cns_ctx_profile": (_INT, [_VP, _INT]),
     "cns_ctx_set_fused": (_INT, [_VP, _INT]),
     "cns_ctx_profile_read": (_INT, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "cns_batch_create": (_INT, [C.POINTER(_VP), C.c_size_t, _INT, _INT]),
+    "cns_batch_destroy": (_INT, [_VP]),
+    "cns_batch_box_bytes": (C.c_size_t, [_INT, _INT]),
+    "cns_batch_add": (_INT, [_VP, _INT, _DBL, _DBL, _DBL, _DBL, _VP, C.POINTER(_INT)]),
+    "cns_batch_remove": (_INT, [_VP, _INT]),
+    "cns_batch_read": (_INT, [_VP, _INT, _VP]),
+    "cns_batch_step": (_INT, [_VP, _INT]),
+    "cns_batch_query": (_INT, [_VP, _INT, C.POINTER(_INT), C.POINTER(C.c_longlong),
+                               C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_INT)]),
+    "cns_batch_set_tile": (_INT, [_VP, _INT, _INT, _INT]),
+    "cns_batch_get_tile": (_INT, [_VP, _INT, C.POINTER(_INT), C.POINTER(_INT)]),
+    "cns_batch_stats": (_INT, [_VP, C.POINTER(C.c_size_t), C.POINTER(_INT), C.POINTER(_INT)]),
 }
 
 PROF_KINDS = ("fill", "ctoprim", "diffterm", "hypterm", "rk", "hypterm_rk", "fill_ctoprim", "other")
@@ -97,5 +117,7 @@ def check(rc: int):
         msg = load().cns_last_error_string().decode(errors="replace")
         if rc == CNS_ENUMERIC:
             raise CnsNumericError(rc, msg)
+        if rc == CNS_EBUDGET:
+            raise CnsBudgetError(rc, msg)
         raise CnsError(rc, msg)
     return rc

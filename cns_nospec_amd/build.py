@@ -21,8 +21,8 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libcns_hip.so")
 ARCH = "gfx950"
 
-SOURCES = ["cns_kernels.hip", "cns_stencil.hip", "cns_abi.hip", "cns_batch.hip"]
-HEADERS = ["cns_common.h", "cns_kernels.h", "cns_batch.h"]
+SOURCES = ["cns_kernels.hip", "cns_stencil.hip", "cns_abi.hip", "cns_batch.hip",
+           "cns_batch_engine.hip"]
 
 
 def hipcc() -> str:

@@ -13,15 +13,17 @@
 #include <vector>
 
 #include "cns.h"
+#include "cns_host.h"
 #include "cns_kernels.h"
 
-using namespace cns;
+namespace cns {
 
 namespace {
-
 thread_local hipStream_t t_stream = nullptr;
 thread_local std::string t_err;
-int g_stencil_impl = CNS_IMPL_TILED;
+}  // namespace
+
+hipStream_t current_stream() { return t_stream; }
 
 int fail(int code, const char *fmt, ...)
 {
@@ -34,11 +36,15 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIPCHK(call)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess) return fail(CNS_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
+}  // namespace cns
+
+using namespace cns;
+
+#define HIPCHK(call) CNS_HIPCHK(call)
+
+namespace {
+
+int g_stencil_impl = CNS_IMPL_TILED;
 
 // Geometry from the ghost-shifted bounds of the reference (lo = ng, hi = n-1+ng).
 int geom_from_bounds(const int lo[3], const int hi[3], int ng, Geom *g)

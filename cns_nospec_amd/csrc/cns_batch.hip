@@ -1,0 +1,690 @@
+// cns_batch.hip -- kernels of the batched engine: many independent periodic boxes
+// of uneven edges advanced together, one persistent launch per RK stage.
+//
+// Per stage, one kernel walks a tile work list (tiles of every active box).  For
+// each tile it does, in one pass and without ghost storage (halos are gathered
+// with periodic index wrap): diffterm and hypterm of the stage state, the SSP-RK3
+// update of each cell in FP64, and ctoprim of the updated cell, whose primitives
+// the next stage reads (so no separate ctoprim pass exists after start-up).  In
+// stage 3 it also folds the Courant number of U^{n+1} into the box's accumulator;
+// a small finalize kernel turns that into the next step's dt.  A step is thus four
+// launches and is captured once into a hipGraph.
+//
+// Flux arithmetic type FT: double reproduces the plain path (and the reference)
+// bit for bit; float is the FP32-flux mode, in which only the stencil arithmetic
+// of hypterm and diffterm runs in FP32 -- U, the RK update, ctoprim's p/T recovery
+// and the dt reduction stay FP64.  In FP32 mode the pressure is carried as
+// p - pref (pref = the box's initial pressure at cell 0) so that the pressure-
+// gradient differences keep FP32's relative precision on the perturbation
+// instead of on the O(1) mean; the energy flux adds pref back.
+#include "cns_batch_dev.h"
+#include "cns_stencil_ops.h"
+
+namespace cns {
+
+namespace {
+
+constexpr int kLdsCells = 864;  // max (TY+8)*(TX+8) over TX in {16, 32, 64}
+
+// Pointers read from a box record are generic; casting them to the global
+// address space lets the compiler address through a scalar base (global_load
+// saddr) instead of a per-lane 64-bit flat address.
+#define GAS __attribute__((address_space(1)))
+
+// Global array indexed by an unsigned 32-bit cell offset, turned into a 32-bit
+// byte offset from a uniform base, so that each access is one scalar-base
+// global_load/store with a 32-bit lane offset (a box component is < 4 GiB).
+template <typename T>
+struct GArr {
+    T GAS *p;
+    __device__ __forceinline__ T GAS &operator[](unsigned i) const
+    {
+        return *(T GAS *)((char GAS *)p + i * (unsigned)sizeof(T));
+    }
+    __device__ __forceinline__ GArr operator+(unsigned n) const
+    {
+        return GArr{(T GAS *)((char GAS *)p + n * (unsigned)sizeof(T))};
+    }
+};
+template <typename T>
+__device__ __forceinline__ GArr<T> garr(void *p)
+{
+    return GArr<T>{(T GAS *)p};
+}
+
+// Keeps the scheduler from hoisting the loads of later groups of a long
+// straight-line block above earlier ones (bounds the live registers).
+#define FENCE() __builtin_amdgcn_sched_barrier(0)
+
+__device__ __forceinline__ int wrapc(int a, int e) { return a < 0 ? a + e : (a >= e ? a - e : a); }
+
+// ctoprim of one cell, cpu/ctoprim.cpp:31-40.
+__device__ __forceinline__ void prim_cell(double r, double mx, double my, double mz, double E,
+                                          double &u, double &v, double &w, double &p, double &T)
+{
+    const double rhoinv = 1.0E0 / r;
+    u = mx * rhoinv;
+    v = my * rhoinv;
+    w = mz * rhoinv;
+    const double eint = E * rhoinv - 0.5E0 * ((u * u) + (v * v) + (w * w));
+    p = (GAMMA - 1.0E0) * eint * r;
+    T = eint / CV;
+}
+
+// Courant number of one cell, cpu/ctoprim.cpp:52-59, as the bits of its max.
+__device__ __forceinline__ unsigned long long cour_bits(double r, double u, double v, double w,
+                                                        double p, const double dx[3])
+{
+    const double c = sqrt(GAMMA * p / r);
+    unsigned long long b = courno_bits((c + fabs(u)) / dx[0]);
+    const unsigned long long by = courno_bits((c + fabs(v)) / dx[1]);
+    const unsigned long long bz = courno_bits((c + fabs(w)) / dx[2]);
+    b = b > by ? b : by;
+    return b > bz ? b : bz;
+}
+
+__device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v)
+{
+    __shared__ unsigned long long red[kBatchThreads / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off);
+        v = v > o ? v : o;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned long long m = red[0];
+#pragma unroll
+    for (int w = 1; w < kBatchThreads / 64; w++) m = m > red[w] ? m : red[w];
+    return m;
+}
+
+// ------------------------------------------------------------------------ init
+// Primitives of U^0 into Q[0], its Courant maximum into acc, and pref.
+template <typename FT>
+__global__ void __launch_bounds__(kBatchThreads)
+batch_init_kernel(BoxDev *__restrict__ boxes, int b)
+{
+    BoxDev &B = boxes[b];
+    const int e = B.e;
+    const int64_t e3 = (int64_t)e * e * e;
+    const double *U = B.U;
+    FT *q = (FT *)B.Q[0];
+    double pref = 0.0;
+    if (sizeof(FT) == 4) {
+        double u0, v0, w0, T0;
+        prim_cell(U[0], U[e3], U[2 * e3], U[3 * e3], U[4 * e3], u0, v0, w0, pref, T0);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) B.pref = pref;
+    const double dx[3] = {B.dx[0], B.dx[1], B.dx[2]};
+    unsigned long long cmax = 0ull;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < e3;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const double r = U[c];
+        double u, v, w, p, T;
+        prim_cell(r, U[c + e3], U[c + 2 * e3], U[c + 3 * e3], U[c + 4 * e3], u, v, w, p, T);
+        q[c] = (FT)u;
+        q[c + e3] = (FT)v;
+        q[c + 2 * e3] = (FT)w;
+        q[c + 3 * e3] = (FT)(p - pref);
+        q[c + 4 * e3] = (FT)T;
+        const unsigned long long cb = cour_bits(r, u, v, w, p, dx);
+        cmax = cmax > cb ? cmax : cb;
+    }
+    const unsigned long long m = block_max_u64(cmax);
+    if (threadIdx.x == 0) atomicMax(&B.acc, m);
+}
+
+// -------------------------------------------------------------------- finalize
+// courno = max(acc, 1e-50), dt = cfl/courno (cpu/advance.cpp:52-58); flag a
+// non-finite Courant number; clear the accumulator for the next stage 3.
+__global__ void batch_finalize_kernel(BoxDev *__restrict__ boxes, const int *__restrict__ ids,
+                                      int n, int count_step)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    BoxDev &B = boxes[ids[t]];
+    const double c = __longlong_as_double((long long)B.acc);
+    double courno = c > 1.0E-50 ? c : 1.0E-50;
+    if (!(c == c)) courno = c;
+    B.courno = courno;
+    B.dt = B.cfl / courno;
+    if (!(courno <= 1.7976931348623157e308)) B.err = 1;
+    B.acc = 0ull;
+    if (count_step) B.steps += 1;
+}
+
+// ----------------------------------------------------------------------- tiles
+// A tile is a TX x TY column block of one box (TX = 2^LTX, TX*TY = 256 threads)
+// marching planes z0 .. z0+nz-1.  Both tile bodies are templates on the width so
+// that every LDS neighbour offset is a compile-time immediate.  LDS cell (r, c)
+// holds box cell (x0+c-4, y0+r-4) wrapped periodically; the lane (tx, ty) owns
+// (r, c) = (ty+4, tx+4).  Lanes whose cell lies past the box edge (partial tiles)
+// still stage and derive values (their wrapped cells are halo for others) but
+// write nothing.
+struct TileGeo {
+    int e;
+    unsigned e2, e3, own;  // cell offsets within one component fit 32 bits (edge <= 512)
+    int xo, yo;
+    bool active;
+};
+
+template <int LTX>
+__device__ __forceinline__ TileGeo tile_geo(int e, int x0, int y0)
+{
+    constexpr int TXr = 1 << LTX;
+    TileGeo t;
+    t.e = e;
+    t.e2 = e * e;
+    t.e3 = t.e2 * e;
+    const int tx = threadIdx.x & (TXr - 1), ty = threadIdx.x >> LTX;
+    t.xo = (x0 + tx) % e;
+    t.yo = (y0 + ty) % e;
+    t.active = (x0 + tx < e) && (y0 + ty < e);
+    t.own = t.yo * e + t.xo;
+    return t;
+}
+
+// Halo cells of an LDS plane: 8*TY x-halo cells (rows 4..TY+3, cols 0..3 and
+// TX+4..TX+7), 8*TX y-halo cells (rows 0..3 and TY+4..TY+7, cols 4..TX+3), 64
+// corners.  Each also gives the wrapped in-plane offset gy*e + gx.
+template <int LTX>
+__device__ __forceinline__ void xhalo(int h, int x0, int y0, int e, int &l, int &ho)
+{
+    constexpr int TXr = 1 << LTX, P = TXr + 8;
+    const int r = 4 + (h >> 3), cc = h & 7, c = cc < 4 ? cc : TXr + cc;
+    l = r * P + c;
+    ho = ((y0 + r - 4) % e) * e + wrapc((x0 + c - 4) % e, e);
+}
+template <int LTX>
+__device__ __forceinline__ void yhalo(int h, int x0, int y0, int e, int &l, int &ho)
+{
+    constexpr int TXr = 1 << LTX, TYr = kBatchThreads >> LTX, P = TXr + 8;
+    const int rr = h >> LTX, c = 4 + (h & (TXr - 1)), r = rr < 4 ? rr : TYr + rr;
+    l = r * P + c;
+    ho = wrapc((y0 + r - 4) % e, e) * e + (x0 + c - 4) % e;
+}
+template <int LTX>
+__device__ __forceinline__ void corner(int h, int x0, int y0, int e, int &l, int &ho)
+{
+    constexpr int TXr = 1 << LTX, TYr = kBatchThreads >> LTX, P = TXr + 8;
+    const int rr = h >> 3, cc = h & 7;
+    const int r = rr < 4 ? rr : TYr + rr, c = cc < 4 ? cc : TXr + cc;
+    l = r * P + c;
+    ho = wrapc((y0 + r - 4) % e, e) * e + wrapc((x0 + c - 4) % e, e);
+}
+
+// ------------------------------------------------------------------ diffterm tile
+// cpu/diffterm.cpp:22-320 on the stage's primitives; D (mx, my, mz, E components;
+// D_rho = 0 is implicit) to the box's D buffer in the flux type.  Same three LDS
+// phases per plane as the plain tiled diffterm.
+template <typename FT, int LTX>
+__device__ __forceinline__ void diff_tile(BoxDev *__restrict__ boxes, int b, int x0, int y0, int z0,
+                                          int nz, int stage, FT *lds)
+{
+    using KK = K<FT>;
+    constexpr int TXr = 1 << LTX, TYr = kBatchThreads >> LTX, P = TXr + 8;
+    FT *const s_u = lds, *const s_v = lds + kLdsCells, *const s_w = lds + 2 * kLdsCells;
+    FT *const s_T = lds + 3 * kLdsCells, *const s_ux = lds + 4 * kLdsCells;
+    FT *const s_vy = lds + 5 * kLdsCells, *const s_wz = lds + 6 * kLdsCells;
+    FT *const s_lu = lds + 7 * kLdsCells, *const s_lv = lds + 8 * kLdsCells;
+    const int tid = threadIdx.x;
+    const BoxDev &B = boxes[b];
+    const TileGeo t = tile_geo<LTX>(B.e, x0, y0);
+    const int e = t.e;
+    const int e2 = t.e2, e3 = t.e3, own = t.own;
+    const int me = ((tid >> LTX) + 4) * P + (tid & (TXr - 1)) + 4;
+    const GArr<FT> qi = garr<FT>(B.Q[stage - 1]);
+    const GArr<FT> q_u = qi, q_v = qi + e3, q_w = qi + 2 * e3, q_t = qi + 4 * e3;
+    const GArr<FT> d_mx = garr<FT>(B.D), d_my = d_mx + e3, d_mz = d_mx + 2 * e3, d_E = d_mx + 3 * e3;
+    const FT dxi0 = (FT)B.dxi[0], dxi1 = (FT)B.dxi[1], dxi2 = (FT)B.dxi[2];
+    const FT dxi0s = (FT)(B.dxi[0] * B.dxi[0]), dxi1s = (FT)(B.dxi[1] * B.dxi[1]),
+             dxi2s = (FT)(B.dxi[2] * B.dxi[2]);
+    const FT eta = (FT)B.eta, alam = (FT)B.alam;
+
+    FT wu[9], wv[9], ww[9], wt[9], wux[9], wvy[9];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int o = wrapc(z0 - 4 + m, e) * e2 + own;
+        wu[m] = q_u[o]; wv[m] = q_v[o]; ww[m] = q_w[o]; wt[m] = q_t[o];
+    }
+#define LUX(mm) s_lu[me + (mm)]
+#define LVY(mm) s_lv[me + (mm) * P]
+    // ux, vy of the first 8 window planes (rolled: its barriers would otherwise
+    // keep every plane's loads live)
+#pragma unroll 1
+    for (int m = 0; m < 8; m++) {
+        const int pl = wrapc(z0 - 4 + m, e) * e2;
+        __syncthreads();
+        s_lu[me] = q_u[pl + own];
+        s_lv[me] = q_v[pl + own];
+        for (int h = tid; h < 8 * TYr; h += kBatchThreads) {
+            int l, ho;
+            xhalo<LTX>(h, x0, y0, e, l, ho);
+            s_lu[l] = q_u[pl + ho];
+        }
+        for (int h = tid; h < 8 * TXr; h += kBatchThreads) {
+            int l, ho;
+            yhalo<LTX>(h, x0, y0, e, l, ho);
+            s_lv[l] = q_v[pl + ho];
+        }
+        __syncthreads();
+        const FT uxm = KD1(LUX) * dxi0, vym = KD1(LVY) * dxi1;
+#pragma unroll
+        for (int mm = 0; mm < 7; mm++) {
+            wux[mm] = wux[mm + 1];
+            wvy[mm] = wvy[mm + 1];
+        }
+        wux[7] = uxm;
+        wvy[7] = vym;
+    }
+
+    for (int k = z0; k < z0 + nz; k++) {
+        const int cp = k * e2, lp = wrapc(k + 4, e) * e2;
+        {
+            const int o = lp + own;
+            wu[8] = q_u[o]; wv[8] = q_v[o]; ww[8] = q_w[o]; wt[8] = q_t[o];
+        }
+        // phase A: lead-plane u (x halo), v (y halo); centre u, v (square), w, T (cross)
+        __syncthreads();
+        s_lu[me] = wu[8]; s_lv[me] = wv[8];
+        s_u[me] = wu[4]; s_v[me] = wv[4]; s_w[me] = ww[4]; s_T[me] = wt[4];
+        for (int h = tid; h < 8 * TYr; h += kBatchThreads) {
+            int l, ho;
+            xhalo<LTX>(h, x0, y0, e, l, ho);
+            const int o = cp + ho;
+            s_u[l] = q_u[o]; s_v[l] = q_v[o]; s_w[l] = q_w[o]; s_T[l] = q_t[o];
+            s_lu[l] = q_u[lp + ho];
+        }
+        for (int h = tid; h < 8 * TXr; h += kBatchThreads) {
+            int l, ho;
+            yhalo<LTX>(h, x0, y0, e, l, ho);
+            const int o = cp + ho;
+            s_u[l] = q_u[o]; s_v[l] = q_v[o]; s_w[l] = q_w[o]; s_T[l] = q_t[o];
+            s_lv[l] = q_v[lp + ho];
+        }
+        if (tid < 64) {
+            int l, ho;
+            corner<LTX>(tid, x0, y0, e, l, ho);
+            s_u[l] = q_u[cp + ho];
+            s_v[l] = q_v[cp + ho];
+        }
+        __syncthreads();
+        // phase B: lead ux, vy into the windows; centre ux (y halo), vy (x halo), wz (cross)
+        wux[8] = KD1(LUX) * dxi0;
+        wvy[8] = KD1(LVY) * dxi1;
+        s_ux[me] = wux[4];
+        s_vy[me] = wvy[4];
+#define WWZ(mm) ww[4 + (mm)]
+        s_wz[me] = KD1(WWZ) * dxi2;
+#undef WWZ
+        for (int h = tid; h < 8 * TXr; h += kBatchThreads) {
+            int l, ho;
+            yhalo<LTX>(h, x0, y0, e, l, ho);
+#define SUX(mm) s_u[l + (mm)]
+            s_ux[l] = KD1(SUX) * dxi0;
+#undef SUX
+        }
+        for (int h = tid; h < 8 * TYr; h += kBatchThreads) {
+            int l, ho;
+            xhalo<LTX>(h, x0, y0, e, l, ho);
+#define SVY(mm) s_v[l + (mm) * P]
+            s_vy[l] = KD1(SVY) * dxi1;
+#undef SVY
+        }
+        for (int h = tid; h < 8 * TYr + 8 * TXr; h += kBatchThreads) {
+            int l, ho;
+            if (h < 8 * TYr) xhalo<LTX>(h, x0, y0, e, l, ho);
+            else yhalo<LTX>(h - 8 * TYr, x0, y0, e, l, ho);
+#define GWZ(mm) q_w[wrapc(k + (mm), e) * e2 + ho]
+            s_wz[l] = KD1(GWZ) * dxi2;
+#undef GWZ
+        }
+        __syncthreads();
+        // phase C (cpu/diffterm.cpp:155-311)
+        if (t.active) {
+#define CUx(m) s_u[me + (m)]
+#define CUy(m) s_u[me + (m) * P]
+#define CUz(m) wu[4 + (m)]
+#define CVx(m) s_v[me + (m)]
+#define CVy(m) s_v[me + (m) * P]
+#define CVz(m) wv[4 + (m)]
+#define CWx(m) s_w[me + (m)]
+#define CWy(m) s_w[me + (m) * P]
+#define CWz(m) ww[4 + (m)]
+#define CTx(m) s_T[me + (m)]
+#define CTy(m) s_T[me + (m) * P]
+#define CTz(m) wt[4 + (m)]
+#define VYx(m) s_vy[me + (m)]
+#define WZx(m) s_wz[me + (m)]
+#define UXy(m) s_ux[me + (m) * P]
+#define WZy(m) s_wz[me + (m) * P]
+#define UXz(m) wux[4 + (m)]
+#define VYz(m) wvy[4 + (m)]
+            const int zo = cp + own;
+            const FT uxx = KD2(CUx) * dxi0s, uyy = KD2(CUy) * dxi1s, uzz = KD2(CUz) * dxi2s;
+            const FT vyx = KD1(VYx) * dxi0, wzx = KD1(WZx) * dxi0;
+            const FT dmx = eta * (KK::FourThirds * uxx + uyy + uzz + KK::OneThird * (vyx + wzx));
+            FENCE();
+            const FT vxx = KD2(CVx) * dxi0s, vyy = KD2(CVy) * dxi1s, vzz = KD2(CVz) * dxi2s;
+            const FT uxy = KD1(UXy) * dxi1, wzy = KD1(WZy) * dxi1;
+            const FT dmy = eta * (vxx + KK::FourThirds * vyy + vzz + KK::OneThird * (uxy + wzy));
+            FENCE();
+            const FT wxx = KD2(CWx) * dxi0s, wyy = KD2(CWy) * dxi1s, wzz = KD2(CWz) * dxi2s;
+            const FT uxz = KD1(UXz) * dxi2, vyz = KD1(VYz) * dxi2;
+            const FT dmz = eta * (wxx + wyy + KK::FourThirds * wzz + KK::OneThird * (uxz + vyz));
+            FENCE();
+            const FT txx = KD2(CTx) * dxi0s, tyy = KD2(CTy) * dxi1s, tzz = KD2(CTz) * dxi2s;
+            FENCE();
+            const FT ux = wux[4], vy = wvy[4], wz = s_wz[me];
+            const FT uy = KD1(CUy) * dxi1, uz = KD1(CUz) * dxi2;
+            const FT vx = KD1(CVx) * dxi0, vz = KD1(CVz) * dxi2;
+            const FT wx = KD1(CWx) * dxi0, wy = KD1(CWy) * dxi1;
+            const FT divu = KK::TwoThirds * (ux + vy + wz);
+            const FT tauxx = KK::Two * ux - divu, tauyy = KK::Two * vy - divu;
+            const FT tauzz = KK::Two * wz - divu;
+            const FT tauxy = uy + vx, tauxz = uz + wx, tauyz = vz + wy;
+            FT mech = tauxx * ux + tauyy * vy + tauzz * wz + (tauxy * tauxy) + (tauxz * tauxz)
+                      + (tauyz * tauyz);
+            mech = eta * mech + dmx * wu[4] + dmy * wv[4] + dmz * ww[4];
+            d_mx[zo] = dmx;
+            d_my[zo] = dmy;
+            d_mz[zo] = dmz;
+            d_E[zo] = alam * (txx + tyy + tzz) + mech;
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            wu[m] = wu[m + 1]; wv[m] = wv[m + 1]; ww[m] = ww[m + 1]; wt[m] = wt[m + 1];
+            wux[m] = wux[m + 1]; wvy[m] = wvy[m + 1];
+        }
+    }
+#undef LUX
+#undef LVY
+}
+
+// ------------------------------------------------------------- hypterm+update tile
+// cpu/hypterm.cpp:11-209 on the stage state, then per cell the SSP-RK3 update in
+// FP64 with the tile's D (cpu/advance.cpp:82, :114-116, :148-150), ctoprim of the
+// result into the next stage's primitives, and in stage 3 the Courant maximum.
+// Returns the lane's Courant maximum (bits; stage 3 only).
+template <typename FT, int LTX>
+__device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxes, int b, int x0,
+                                                       int y0, int z0, int nz, int stage, FT *lds)
+{
+    using KK = K<FT>;
+    constexpr int TXr = 1 << LTX, TYr = kBatchThreads >> LTX, P = TXr + 8;
+    FT *const s_mx = lds, *const s_my = lds + kLdsCells, *const s_mz = lds + 2 * kLdsCells;
+    FT *const s_E = lds + 3 * kLdsCells, *const s_p = lds + 4 * kLdsCells;
+    FT *const s_u = lds + 5 * kLdsCells, *const s_v = lds + 6 * kLdsCells;
+    const int tid = threadIdx.x;
+    const BoxDev &B = boxes[b];
+    const TileGeo t = tile_geo<LTX>(B.e, x0, y0);
+    const int e = t.e;
+    const int e2 = t.e2, e3 = t.e3, own = t.own;
+    const int me = ((tid >> LTX) + 4) * P + (tid & (TXr - 1)) + 4;
+    const double *cur = stage == 1 ? B.U : (stage == 2 ? B.S1 : B.S2);
+    double *out = stage == 1 ? B.S1 : (stage == 2 ? B.S2 : B.U);
+    // per-component uniform bases: every access is scalar base + 32-bit offset
+    const GArr<double> c_r = garr<double>((void *)cur), c_mx = c_r + e3, c_my = c_r + 2 * e3,
+                       c_mz = c_r + 3 * e3, c_E = c_r + 4 * e3;
+    const GArr<double> b_0 = garr<double>(B.U), o_0 = garr<double>(out);
+    const GArr<FT> qi = garr<FT>(B.Q[stage - 1]), qo = garr<FT>(B.Q[stage % 3]);
+    const GArr<FT> din = garr<FT>(B.D);
+    const GArr<FT> q_u = qi, q_v = qi + e3, q_w = qi + 2 * e3, q_p = qi + 3 * e3;
+    const double dt = B.dt;
+    const double ra = stage == 1 ? 0.0E0 : (stage == 2 ? ThreeQuarters : OneThird);
+    const double rb = stage == 1 ? 1.0E0 : (stage == 2 ? OneQuarter : TwoThirds);
+    const double dxd[3] = {B.dx[0], B.dx[1], B.dx[2]};
+    const double pref = B.pref;
+    const FT dxi0 = (FT)B.dxi[0], dxi1 = (FT)B.dxi[1], dxi2 = (FT)B.dxi[2], prefF = (FT)pref;
+
+    FT ww[9], wp[9];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int o = wrapc(z0 - 4 + m, e) * e2 + own;
+        ww[m] = q_w[o]; wp[m] = q_p[o];
+    }
+    unsigned long long cmax = 0ull;
+    for (int k = z0; k < z0 + nz; k++) {
+        const int cp = k * e2;
+        const int zo = cp + own;
+        {
+            const int o = wrapc(k + 4, e) * e2 + own;
+            ww[8] = q_w[o]; wp[8] = q_p[o];
+        }
+        const FT u_own = q_u[zo], v_own = q_v[zo];
+        const FT mx_own = (FT)c_mx[zo], my_own = (FT)c_my[zo], mz_own = (FT)c_mz[zo], E_own = (FT)c_E[zo];
+        __syncthreads();
+        s_mx[me] = mx_own; s_my[me] = my_own; s_mz[me] = mz_own; s_E[me] = E_own; s_p[me] = wp[4];
+        s_u[me] = u_own; s_v[me] = v_own;
+        for (int h = tid; h < 8 * TYr; h += kBatchThreads) {
+            int l, ho;
+            xhalo<LTX>(h, x0, y0, e, l, ho);
+            const int o = cp + ho;
+            s_mx[l] = (FT)c_mx[o]; s_my[l] = (FT)c_my[o]; s_mz[l] = (FT)c_mz[o]; s_E[l] = (FT)c_E[o];
+            s_p[l] = q_p[o]; s_u[l] = q_u[o];
+        }
+        for (int h = tid; h < 8 * TXr; h += kBatchThreads) {
+            int l, ho;
+            yhalo<LTX>(h, x0, y0, e, l, ho);
+            const int o = cp + ho;
+            s_mx[l] = (FT)c_mx[o]; s_my[l] = (FT)c_my[o]; s_mz[l] = (FT)c_mz[o]; s_E[l] = (FT)c_E[o];
+            s_p[l] = q_p[o]; s_v[l] = q_v[o];
+        }
+        __syncthreads();
+        if (t.active) {
+            FT F[5];
+#define XMX(m) s_mx[me + (m)]
+#define XMY(m) s_my[me + (m)]
+#define XMZ(m) s_mz[me + (m)]
+#define XE(m) s_E[me + (m)]
+#define XP(m) s_p[me + (m)]
+#define XPF(m) (s_p[me + (m)] + prefF)
+#define XU(m) s_u[me + (m)]
+            F[IRHO] = -(KD1(XMX)) * dxi0;
+            F[IMX] = -(KHMOMP(XMX, XU, XP)) * dxi0;
+            F[IMY] = -(KHMOM(XMY, XU)) * dxi0;
+            F[IMZ] = -(KHMOM(XMZ, XU)) * dxi0;
+            F[IENE] = -(KHENE(XE, XU, XPF)) * dxi0;
+            FENCE();
+#define YMX(m) s_mx[me + (m) * P]
+#define YMY(m) s_my[me + (m) * P]
+#define YMZ(m) s_mz[me + (m) * P]
+#define YE(m) s_E[me + (m) * P]
+#define YP(m) s_p[me + (m) * P]
+#define YPF(m) (s_p[me + (m) * P] + prefF)
+#define YV(m) s_v[me + (m) * P]
+            F[IRHO] = F[IRHO] - (KD1(YMY)) * dxi1;
+            F[IMX] = F[IMX] - (KHMOM(YMX, YV)) * dxi1;
+            F[IMY] = F[IMY] - (KHMOMP(YMY, YV, YP)) * dxi1;
+            F[IMZ] = F[IMZ] - (KHMOM(YMZ, YV)) * dxi1;
+            F[IENE] = F[IENE] - (KHENE(YE, YV, YPF)) * dxi1;
+            FENCE();
+#define ZPL(m) (wrapc(k + (m), e) * e2 + own)
+#define ZMX(m) ((FT)c_mx[ZPL(m)])
+#define ZMY(m) ((FT)c_my[ZPL(m)])
+#define ZMZ(m) ((FT)c_mz[ZPL(m)])
+#define ZE(m) ((FT)c_E[ZPL(m)])
+#define ZW(m) ww[4 + (m)]
+#define ZP(m) wp[4 + (m)]
+#define ZPF(m) (wp[4 + (m)] + prefF)
+            F[IRHO] = F[IRHO] - (KD1(ZMZ)) * dxi2;
+            F[IMX] = F[IMX] - (KHMOM(ZMX, ZW)) * dxi2;
+            F[IMY] = F[IMY] - (KHMOM(ZMY, ZW)) * dxi2;
+            F[IMZ] = F[IMZ] - (KHMOMP(ZMZ, ZW, ZP)) * dxi2;
+            F[IENE] = F[IENE] - (KHENE(ZE, ZW, ZPF)) * dxi2;
+            FENCE();
+            // one branch-free form for the three stages: out = ra*base + rb*(cur + dt*(D+F))
+            // with (ra, rb) = (0, 1), (3/4, 1/4), (1/3, 2/3); for stage 1, 0*base + 1*x
+            // is exactly x, so every stage keeps the reference's rounding.
+            double o[5];
+#pragma unroll
+            for (int l = 0; l < NC; l++) {
+                const double dl = l == 0 ? 0.0E0 : (double)din[(l - 1) * e3 + zo];
+                const double df = dl + (double)F[l];
+                o[l] = ra * b_0[l * e3 + zo] + rb * (c_r[l * e3 + zo] + dt * df);
+                o_0[l * e3 + zo] = o[l];
+            }
+            double u, v, w, p, T;
+            prim_cell(o[0], o[1], o[2], o[3], o[4], u, v, w, p, T);
+            qo[zo] = (FT)u;
+            qo[zo + e3] = (FT)v;
+            qo[zo + 2 * e3] = (FT)w;
+            qo[zo + 3 * e3] = (FT)(p - pref);
+            qo[zo + 4 * e3] = (FT)T;
+            if (stage == 3) {
+                const unsigned long long cb = cour_bits(o[0], u, v, w, p, dxd);
+                cmax = cmax > cb ? cmax : cb;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            ww[m] = ww[m + 1];
+            wp[m] = wp[m + 1];
+        }
+    }
+    return cmax;
+}
+
+struct TileArgs {
+    int b, x0, y0, z0, nz;
+};
+
+__device__ __forceinline__ TileArgs read_tile(const TileDev *__restrict__ tiles, int ti)
+{
+    const TileDev tl = tiles[ti];
+    TileArgs a;
+    a.b = __builtin_amdgcn_readfirstlane(tl.box);
+    a.x0 = __builtin_amdgcn_readfirstlane(tl.x0);
+    a.y0 = __builtin_amdgcn_readfirstlane(tl.y0);
+    a.z0 = __builtin_amdgcn_readfirstlane(tl.z0);
+    a.nz = __builtin_amdgcn_readfirstlane(tl.nz);
+    return a;
+}
+
+// Persistent kernels over one width class of the tile list.  Workgroup b takes
+// tiles v, v+G, v+2G, ... with v the XCD-contiguous renumbering of b, so that in
+// each round the tiles one XCD works on are neighbours in the list (the list
+// keeps a box's tiles together), and their halos are shared in that XCD's L2.
+__device__ __forceinline__ int xcd_rank(int b, int nb)
+{
+    const int xcd = b & 7, local = b >> 3;
+    const int per = nb >> 3, rem = nb & 7;
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+template <typename FT, int LTX>
+__global__ void __launch_bounds__(kBatchThreads, 2)
+batch_diff_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles, int stage)
+{
+    __shared__ FT lds[9 * kLdsCells];
+    for (int ti = xcd_rank(blockIdx.x, gridDim.x); ti < ntiles; ti += gridDim.x) {
+        const TileArgs a = read_tile(tiles, ti);
+        diff_tile<FT, LTX>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, stage, lds);
+    }
+}
+
+template <typename FT, int LTX>
+__global__ void __launch_bounds__(kBatchThreads, 2)
+batch_hyp_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles, int stage)
+{
+    __shared__ FT lds[7 * kLdsCells];
+    for (int ti = xcd_rank(blockIdx.x, gridDim.x); ti < ntiles; ti += gridDim.x) {
+        const TileArgs a = read_tile(tiles, ti);
+        const unsigned long long cm = hyp_tile<FT, LTX>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, stage, lds);
+        if (stage == 3) {
+            const unsigned long long m = block_max_u64(cm);
+            if (threadIdx.x == 0) atomicMax(&boxes[a.b].acc, m);
+        }
+    }
+}
+
+template <typename FT, int LTX>
+hipError_t stage_class(BoxDev *boxes, const TileDev *tiles, int n, int grid, int stage, bool diff,
+                       hipStream_t s)
+{
+    if (diff)
+        batch_diff_kernel<FT, LTX><<<grid, kBatchThreads, 0, s>>>(boxes, tiles, n, stage);
+    else
+        batch_hyp_kernel<FT, LTX><<<grid, kBatchThreads, 0, s>>>(boxes, tiles, n, stage);
+    return hipGetLastError();
+}
+
+template <typename FT, int LTX>
+int class_occupancy()
+{
+    int pd = 1, ph = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pd, batch_diff_kernel<FT, LTX>, kBatchThreads, 0) != hipSuccess)
+        pd = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&ph, batch_hyp_kernel<FT, LTX>, kBatchThreads, 0) != hipSuccess)
+        ph = 1;
+    const int per = pd < ph ? pd : ph;
+    return per < 1 ? 1 : per;
+}
+
+}  // namespace
+
+hipError_t launch_batch_init(BoxDev *boxes, int box, int e, bool fp32, hipStream_t s)
+{
+    const int64_t e3 = (int64_t)e * e * e;
+    int grid = (int)((e3 + kBatchThreads * 4 - 1) / (kBatchThreads * 4));
+    grid = grid < 1 ? 1 : (grid > 4096 ? 4096 : grid);
+    if (fp32)
+        batch_init_kernel<float><<<grid, kBatchThreads, 0, s>>>(boxes, box);
+    else
+        batch_init_kernel<double><<<grid, kBatchThreads, 0, s>>>(boxes, box);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_finalize(BoxDev *boxes, const int *ids, int n, int count_step, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    batch_finalize_kernel<<<(n + 255) / 256, 256, 0, s>>>(boxes, ids, n, count_step);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int start[3],
+                              const int count[3], const int grid[3], int stage, bool fp32, hipStream_t s)
+{
+    for (int pass = 0; pass < 2; pass++) {  // all diffterm launches, then all hypterm launches
+        const bool diff = pass == 0;
+        for (int c = 0; c < 3; c++) {
+            const int n = count[c];
+            if (n <= 0) continue;
+            const int g = grid[c] < n ? grid[c] : n;
+            const TileDev *t = tiles + start[c];
+            hipError_t e;
+            if (fp32)
+                e = c == 0 ? stage_class<float, 4>(boxes, t, n, g, stage, diff, s)
+                  : c == 1 ? stage_class<float, 5>(boxes, t, n, g, stage, diff, s)
+                           : stage_class<float, 6>(boxes, t, n, g, stage, diff, s);
+            else
+                e = c == 0 ? stage_class<double, 4>(boxes, t, n, g, stage, diff, s)
+                  : c == 1 ? stage_class<double, 5>(boxes, t, n, g, stage, diff, s)
+                           : stage_class<double, 6>(boxes, t, n, g, stage, diff, s);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+void batch_class_grids(bool fp32, int grid[3])
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+    }
+    if (fp32) {
+        grid[0] = cus * class_occupancy<float, 4>();
+        grid[1] = cus * class_occupancy<float, 5>();
+        grid[2] = cus * class_occupancy<float, 6>();
+    } else {
+        grid[0] = cus * class_occupancy<double, 4>();
+        grid[1] = cus * class_occupancy<double, 5>();
+        grid[2] = cus * class_occupancy<double, 6>();
+    }
+}
+
+}  // namespace cns

@@ -1,0 +1,222 @@
+"""Serving: a request loop that accepts CNS solves as they arrive.
+
+A request is one periodic box (edge, its own seeded sinusoidal state and Mach
+number) to be advanced a number of RK3 steps.  :class:`BatchServer` does
+continuous batching on the batched engine (:mod:`.batch`): requests that arrive
+while others are on the GPU are admitted as soon as the engine's HBM budget has
+room, every resident box advances in the same graph-captured launches, and a box
+leaves as soon as its own step count is reached.  :class:`PlainServer` serves
+the same stream through the plain path (one box at a time via ``cns_advance``,
+dt to the host every step, as the reference's main loop), first come first served.
+
+Both report per-request latency (arrival to result), p50/p99, aggregate
+cell-steps/s, and failures (a request whose state becomes non-physical fails
+alone).
+"""
+from __future__ import annotations
+
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .layout import NC, BoxLayout
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float  # seconds after the stream starts
+    edge: int
+    steps: int
+    seed: int
+    mach: float
+
+
+@dataclass
+class Result:
+    rid: int
+    edge: int
+    steps: int
+    arrival: float
+    admitted: float = math.nan
+    done: float = math.nan
+    failed: bool = False
+    state: object = None  # final U (device tensor) when kept
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+def make_stream(n: int, rate: float, seed: int, edge_range=(24, 128), steps_range=(50, 500),
+                mach_range=(0.05, 0.3)):
+    """n requests with Poisson arrivals at `rate` per second, edge ~ U{edge_range},
+    steps ~ U{steps_range}, Mach ~ U(mach_range), seeded."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n)) if rate > 0 else np.zeros(n)
+    edges = rng.integers(edge_range[0], edge_range[1] + 1, n)
+    steps = rng.integers(steps_range[0], steps_range[1] + 1, n)
+    machs = rng.uniform(mach_range[0], mach_range[1], n)
+    seeds = rng.integers(0, 2 ** 31 - 1, n)
+    return [Request(i, float(t[i]), int(edges[i]), int(steps[i]), int(seeds[i]), float(machs[i]))
+            for i in range(n)]
+
+
+def request_state(req: Request, device="cuda"):
+    """The request's initial ghost-free U (5, e, e, e) on the device."""
+    from .state import GAMMA, _draw_modes
+    import torch
+    e = req.edge
+    rng = np.random.default_rng(req.seed)
+    crd = (torch.arange(e, dtype=torch.float64, device=device) + 0.5) / e
+    c0 = math.sqrt(GAMMA)
+    prim = {}
+    for name, f0, amp in (("rho", 1.0, 0.05), ("u", 0.0, req.mach * c0), ("v", 0.0, req.mach * c0),
+                          ("w", 0.0, req.mach * c0), ("p", 1.0, 0.05)):
+        s = torch.zeros((e, e, e), dtype=torch.float64, device=device)
+        for k, ph, a in _draw_modes(rng, 2, 4):
+            fx = torch.sin(2 * math.pi * int(k[0]) * crd + float(ph[0]))
+            fy = torch.sin(2 * math.pi * int(k[1]) * crd + float(ph[1]))
+            fz = torch.sin(2 * math.pi * int(k[2]) * crd + float(ph[2]))
+            s.add_(float(a) * fz[:, None, None] * fy[None, :, None] * fx[None, None, :])
+        s.mul_(amp / float(s.abs().max())).add_(f0)
+        prim[name] = s
+    rho, u, v, w, p = prim["rho"], prim["u"], prim["v"], prim["w"], prim["p"]
+    return torch.stack([rho, rho * u, rho * v, rho * w,
+                        p / (GAMMA - 1.0) + 0.5 * rho * (u * u + v * v + w * w)]).contiguous()
+
+
+def summarize(results, wall: float):
+    ok = [r for r in results if not r.failed]
+    lat = np.array([r.latency for r in ok]) if ok else np.array([math.nan])
+    work = sum(r.edge ** 3 * r.steps for r in ok)
+    return {"requests": len(results), "failed": len(results) - len(ok),
+            "cell_steps": int(work), "wall_s": wall,
+            "cell_steps_per_s": work / wall if wall > 0 else math.nan,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean())}
+
+
+class _Clock:
+    """Stream time: real time since start, fast-forwarded over idle gaps when the
+    server has nothing to do (so a sparse stream does not waste the GPU's time
+    budget; latency is still measured in stream time)."""
+
+    def __init__(self):
+        self.t0 = time.perf_counter()
+        self.skip = 0.0
+
+    def now(self):
+        return time.perf_counter() - self.t0 + self.skip
+
+    def advance_to(self, t):
+        cur = self.now()
+        if t > cur:
+            self.skip += t - cur
+
+
+class BatchServer:
+    """Continuous batching on one BatchEngine."""
+
+    def __init__(self, hbm_budget: int, fp32: bool = True, max_boxes: int = 4096,
+                 max_chunk: int = 16, keep_states=()):
+        from .batch import BatchEngine
+        self.engine = BatchEngine(hbm_budget, fp32=fp32, max_boxes=max_boxes)
+        self.max_chunk = max_chunk
+        self.keep = set(keep_states)
+
+    def run(self, requests):
+        import torch
+        eng = self.engine
+        clock = _Clock()
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = deque()
+        active = {}  # box id -> (request, steps left, result)
+        results = {}
+        while pending or waiting or active:
+            now = clock.now()
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            # admission: FIFO while the budget has room
+            while waiting:
+                req = waiting[0]
+                try:
+                    bid = eng.add(request_state(req).reshape(-1), req.edge)
+                except _lib.CnsBudgetError:
+                    break
+                waiting.popleft()
+                res = Result(req.rid, req.edge, req.steps, req.arrival, admitted=clock.now())
+                results[req.rid] = res
+                active[bid] = [req, req.steps, res]
+            if not active:
+                if pending:
+                    clock.advance_to(pending[0].arrival)
+                    continue
+                if waiting:
+                    raise RuntimeError("a single request does not fit in the engine's budget")
+                break
+            k = min(self.max_chunk, min(v[1] for v in active.values()))
+            eng.step(k)
+            torch.cuda.synchronize()
+            for bid in list(active):
+                active[bid][1] -= k
+            finished = [bid for bid, v in active.items() if v[1] == 0]
+            if finished:
+                q = eng.query(finished)
+                t = clock.now()
+                for bid in finished:
+                    req, _, res = active.pop(bid)
+                    res.done = t
+                    res.failed = bool(q[bid]["err"])
+                    if req.rid in self.keep:
+                        res.state = eng.read(bid).clone()
+                    eng.remove(bid)
+        wall = max((r.done for r in results.values()), default=0.0) - min(
+            (r.arrival for r in results.values()), default=0.0)
+        out = [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
+        return out, summarize(out, wall)
+
+    def close(self):
+        self.engine.close()
+
+
+class PlainServer:
+    """The plain path on the same stream: one box at a time through cns_advance
+    (ghosted layout, dt to the host every step), first come first served."""
+
+    def __init__(self, keep_states=()):
+        self.keep = set(keep_states)
+
+    def run(self, requests):
+        import torch
+        from .ops import Stepper, allocate_variables
+        clock = _Clock()
+        results = []
+        for req in sorted(requests, key=lambda r: r.arrival):
+            clock.advance_to(req.arrival)
+            res = Result(req.rid, req.edge, req.steps, req.arrival, admitted=clock.now())
+            lay = BoxLayout.cube(req.edge)
+            U, Unew, Q, D, F = allocate_variables(lay, "cuda")
+            g, ng, e = lay.dimg, lay.ng, req.edge
+            U.view(NC, g[2], g[1], lay.xg)[:, ng:ng + e, ng:ng + e, ng:ng + e].copy_(request_state(req))
+            st = Stepper(lay)
+            try:
+                for _ in range(req.steps):
+                    st.new_advance(U, Unew, Q, D, F)
+            except _lib.CnsNumericError:
+                res.failed = True
+            torch.cuda.synchronize()
+            res.done = clock.now()
+            if req.rid in self.keep:
+                res.state = U.view(NC, g[2], g[1], lay.xg)[:, ng:ng + e, ng:ng + e, ng:ng + e].clone()
+            st.close()
+            del U, Unew, Q, D, F
+            results.append(res)
+        wall = max(r.done for r in results) - min(r.arrival for r in results)
+        out = sorted(results, key=lambda r: r.rid)
+        return out, summarize(out, wall)
