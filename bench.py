@@ -155,6 +155,30 @@ def cpu_baseline(edge: int = 128, budget_s: float = 15.0):
                       f"{el:.1f} s; the path has no threading (its OpenMP is commented out)"}
 
 
+def kernels_once(edge: int, reps: int = 3):
+    """One warm-up and `reps` launches of every kernel of the fused plain step on an
+    edge^3 box, plus a plain device copy of known size (for calibrating rocprofv3
+    byte counters).  Used as the target of rocprofv3 runs; prints nothing timed."""
+    import torch
+    from cns_nospec_amd import ops
+    from cns_nospec_amd.layout import BoxLayout
+    from cns_nospec_amd.state import device_sinusoid_state
+
+    dev = torch.device("cuda:0")
+    lay = BoxLayout.cube(edge)
+    U, Unew, Q, D, F = ops.allocate_variables(lay, dev)
+    device_sinusoid_state(lay, 2024, dev, kmax=4, out=U)
+    st = ops.Stepper(lay)
+    for _ in range(reps + 1):
+        st.new_advance(U, Unew, Q, D, F)
+    src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)  # 1 GiB
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernels_once": edge, "reps": reps, "copy_bytes_each_way": src.numel() * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +187,8 @@ def main():
     ap.add_argument("--edge", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--kernels-once", action="store_true",
+                    help="run each step kernel a few times and a calibration copy (for rocprofv3)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -177,6 +203,9 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     _lib.load()
+    if args.kernels_once:
+        kernels_once(args.edge)
+        return
 
     main_run = run_plain(args.edge, args.steps, args.warmup)
     kern = main_run["kernels"]

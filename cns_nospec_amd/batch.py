@@ -221,9 +221,10 @@ if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="autotune the batched engine's tile table")
     ap.add_argument("--edges", default="16,24,32,48,64,96,128")
+    ap.add_argument("--out", default=TUNING_FILE)
     args = ap.parse_args()
     edges = [int(x) for x in args.edges.split(",")]
     r32 = autotune(edges, True, verbose=True)
     r64 = autotune(edges, False, verbose=True)
-    save_tuning(r32, r64)
-    print(json.dumps({"saved": TUNING_FILE}))
+    save_tuning(r32, r64, args.out)
+    print(json.dumps({"saved": args.out}))
