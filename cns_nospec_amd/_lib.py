@@ -12,9 +12,9 @@ import re
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libcns_hip.so")
 INCLUDE = os.path.join(os.path.dirname(PKG), "include")
-HEADERS = [os.path.join(INCLUDE, h) for h in ("cns.h", "cns_batch.h")]
+HEADERS = [os.path.join(INCLUDE, h) for h in ("cns.h", "cns_batch.h", "cns_io.h")]
 
-CNS_OK, CNS_EINVAL, CNS_EHIP, CNS_ENOMEM, CNS_ENUMERIC, CNS_EBUDGET = 0, -1, -2, -3, -4, -5
+CNS_OK, CNS_EINVAL, CNS_EHIP, CNS_ENOMEM, CNS_ENUMERIC, CNS_EBUDGET, CNS_EIO = 0, -1, -2, -3, -4, -5, -6
 CNS_IMPL_TILED, CNS_IMPL_DIRECT = 0, 1
 
 _lib = None
@@ -34,6 +34,19 @@ class CnsNumericError(CnsError):
 
 class CnsBudgetError(CnsError):
     """A box does not fit in the batched engine's HBM budget."""
+
+
+class CnsIOError(CnsError):
+    """A reference-format file could not be read or written, or is malformed."""
+
+
+class Config(C.Structure):
+    """struct cns_config (include/cns_io.h): the fields of a general_input file."""
+    _fields_ = [("ng", C.c_int), ("nc", C.c_int), ("ncells", C.c_int),
+                ("lo", C.c_int * 3), ("hi", C.c_int * 3), ("dx", C.c_double * 3),
+                ("cfl", C.c_double), ("eta", C.c_double), ("alam", C.c_double),
+                ("nsteps", C.c_int), ("dt", C.c_double),
+                ("dim", C.c_int * 3), ("dim_g", C.c_int * 3)]
 
 
 def header_symbols(paths=None):
@@ -82,6 +95,11 @@ _SIGS = {
     "cns_batch_set_tile": (_INT, [_VP, _INT, _INT, _INT]),
     "cns_batch_get_tile": (_INT, [_VP, _INT, C.POINTER(_INT), C.POINTER(_INT)]),
     "cns_batch_stats": (_INT, [_VP, C.POINTER(C.c_size_t), C.POINTER(_INT), C.POINTER(_INT)]),
+    "cns_read_general_input": (_INT, [C.c_char_p, C.POINTER(Config)]),
+    "cns_write_general_input": (_INT, [C.c_char_p, C.POINTER(Config)]),
+    "cns_read_field": (_INT, [C.c_char_p, _I3, _INT, _INT, _VP]),
+    "cns_write_field": (_INT, [C.c_char_p, _I3, _INT, _INT, _VP]),
+    "cns_field_len": (C.c_size_t, [_I3, _INT]),
 }
 
 PROF_KINDS = ("fill", "ctoprim", "diffterm", "hypterm", "rk", "hypterm_rk", "fill_ctoprim", "other")
@@ -119,5 +137,7 @@ def check(rc: int):
             raise CnsNumericError(rc, msg)
         if rc == CNS_EBUDGET:
             raise CnsBudgetError(rc, msg)
+        if rc == CNS_EIO:
+            raise CnsIOError(rc, msg)
         raise CnsError(rc, msg)
     return rc

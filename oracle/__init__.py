@@ -130,3 +130,23 @@ class Reference:
         dt = C.c_double(0.0)
         self.lib.ref_new_advance(_p(U), _p(Unew), _p(Q), _p(D), _p(F), C.byref(dt))
         return dt.value
+
+
+def reference_program_available() -> bool:
+    return os.path.exists(BIN_REF)
+
+
+def run_reference_program(workdir: str, timeout: float = 600):
+    """Run the unmodified reference cpu/ program (oracle/_ref/cpu) on
+    workdir/testcases/{general_input,multistep_input}.  It reads ../testcases/
+    relative to its working directory (cpu/main.cpp:21, cpu/helper_functions.cpp:40)
+    and writes ``output`` there, so it runs in workdir/run.  Returns
+    (stdout, path of the output file)."""
+    if not reference_program_available():
+        raise FileNotFoundError(f"{BIN_REF} not built (needs the reference tree)")
+    run = os.path.join(workdir, "run")
+    os.makedirs(run, exist_ok=True)
+    res = subprocess.run([BIN_REF], cwd=run, capture_output=True, text=True, timeout=timeout)
+    if res.returncode != 0:
+        raise RuntimeError(f"reference program failed ({res.returncode}): {res.stderr[-500:]}")
+    return res.stdout, os.path.join(run, "output")

@@ -18,12 +18,17 @@ Fixtures (float64, interior cells, [l][k][j][i] order, AoS-independent):
   sample cells; plus U0.
 * ``cgs_16.npz``       -- the reference's own CGS initial condition
   (fortran90/init_data.f90) at 16^3, U after 3 steps and the dts.
+* ``refio_16.npz``     -- the reference PROGRAM (oracle/_ref/cpu, cpu/main.cpp) run on
+  testcase files holding the U0 of steps_16 and of cgs_16 (ghosts zero): SHA-256 and
+  size of its ``output`` file after 3 steps and its per-step "dt, courno" lines.
+  ``python tests/golden/make_golden.py refio`` rewrites only this one.
 """
 from __future__ import annotations
 
 import hashlib
 import os
 import sys
+import tempfile
 
 import numpy as np
 
@@ -92,8 +97,41 @@ def stats_of(arr):
                      (arr * arr).sum(axis=(1, 2, 3))])
 
 
+REFIO_CASES = (("sin16", "steps_16.npz"), ("cgs16", "cgs_16.npz"))
+REFIO_STEPS = 3
+
+
+def refio_input(fixture):
+    """Ghosted padded U (ghosts zero) holding a fixture's U0, and the box layout."""
+    z = np.load(os.path.join(HERE, fixture))
+    n = int(np.asarray(z["n"]).ravel()[0])
+    lay = BoxLayout.cube(n)
+    U = lay.new_ghosted(NC)
+    lay.interior_of(lay.gview(U, NC))[...] = z["U0"]
+    return lay, U
+
+
+def refio():
+    from cns_nospec_amd import refio as rio
+    out = {"steps": REFIO_STEPS, "cases": np.array([c for c, _ in REFIO_CASES])}
+    for case, fixture in REFIO_CASES:
+        lay, U = refio_input(fixture)
+        work = tempfile.mkdtemp(prefix="refio_")
+        cfg = rio.make_config(lay.n, box_dx(lay.n), CFL, ETA, ALAM, REFIO_STEPS)
+        rio.write_testcase(os.path.join(work, "testcases"), cfg, U)
+        stdout, path = oracle.run_reference_program(work)
+        data = open(path, "rb").read()
+        out[f"{case}_sha256"] = np.array(hashlib.sha256(data).hexdigest())
+        out[f"{case}_size"] = np.array(len(data))
+        out[f"{case}_lines"] = np.array(rio.step_lines(stdout))
+    np.savez_compressed(os.path.join(HERE, "refio_16.npz"), **out)
+
+
 def main():
     os.makedirs(HERE, exist_ok=True)
+    if sys.argv[1:] == ["refio"]:
+        refio()
+        return
     lay16 = BoxLayout.cube(16)
     np.savez_compressed(os.path.join(HERE, "kernels_16.npz"), **kernels(lay16, SEED_16))
     np.savez_compressed(os.path.join(HERE, "kernels_nc.npz"), **kernels(BoxLayout((16, 12, 20)), SEED_NC))
@@ -121,6 +159,7 @@ def main():
     dts, kept, _ = run(lay16, U, 3, keep=(3,))
     np.savez_compressed(os.path.join(HERE, "cgs_16.npz"), n=16, U0=interior_soa(lay16, U, NC),
                         dt=dts, U3=kept[3])
+    refio()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
