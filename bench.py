@@ -50,7 +50,9 @@ def algorithmic_bytes(kind: str, n: int, stage: int = 2) -> float:
         "diffterm": 4 * g3 + 5 * n3,
         "hypterm": 8 * g3 + 5 * n3,
         "rk": rk,
-        "hypterm_rk": 8 * g3 + 5 * n3 + rk,
+        # the fused pass's own minimum: hypterm's reads and F, plus D, the stage base and
+        # rho of the cell in, the stage result out (F is not re-read)
+        "hypterm_rk": 8 * g3 + 21 * n3,
         "fill_ctoprim": 10 * (g3 - n3) + 11 * g3,
     }
     return 8.0 * table[kind]

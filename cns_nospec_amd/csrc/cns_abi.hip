@@ -6,7 +6,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <new>
 #include <string>
@@ -85,19 +87,86 @@ int thread_acc(unsigned long long **out)
     return CNS_OK;
 }
 
+// Device scratch of the free-standing cns_diffterm (the wz field of the pipelined
+// kernel): one grow-only buffer per (thread, device), reused by later calls.
+struct TlsScratch {
+    double *p[64] = {};
+    size_t n[64] = {};
+    ~TlsScratch()
+    {
+        for (auto *q : p)
+            if (q) (void)hipFree(q);
+    }
+};
+thread_local TlsScratch t_scratch;
+
+int thread_scratch(size_t count, double **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(CNS_EINVAL, "device %d out of range", dev);
+    if (t_scratch.n[dev] < count) {
+        if (t_scratch.p[dev]) {
+            HIPCHK(hipStreamSynchronize(t_stream));
+            HIPCHK(hipFree(t_scratch.p[dev]));
+            t_scratch.p[dev] = nullptr;
+            t_scratch.n[dev] = 0;
+        }
+        if (hipMalloc((void **)&t_scratch.p[dev], count * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            t_scratch.p[dev] = nullptr;
+            return fail(CNS_ENOMEM, "hipMalloc of %zu bytes of diffterm scratch failed", count * sizeof(double));
+        }
+        t_scratch.n[dev] = count;
+    }
+    *out = t_scratch.p[dev];
+    return CNS_OK;
+}
+
+// The tiled implementation runs the DMA-pipelined kernels (cns_stencil_pipe.hip)
+// when every array is 16-byte aligned, else the register-staged ones
+// (cns_stencil.hip).  CNS_PIPE=0 in the environment selects the latter always.
+bool pipe_on(std::initializer_list<const void *> ptrs)
+{
+    const char *e = getenv("CNS_PIPE");
+    if (e && atoi(e) == 0) return false;
+    for (const void *p : ptrs)
+        if (p && !pipe_aligned(p)) return false;
+    return true;
+}
+
 hipError_t stencil_hypterm(const Geom &g, const double *cons, const double *q, double *flux,
                            const double dxi[3], hipStream_t s)
 {
     if (g_stencil_impl == CNS_IMPL_DIRECT)
         return launch_hypterm_direct(g, cons, q, flux, dxi[0], dxi[1], dxi[2], s);
+    if (pipe_on({cons, q, flux}))
+        return launch_hypterm_pipe(g, cons, q, flux, dxi[0], dxi[1], dxi[2], nullptr, nullptr,
+                                   nullptr, nullptr, 0, s);
     return launch_hypterm_tiled(g, cons, q, flux, dxi[0], dxi[1], dxi[2], s);
 }
 
+hipError_t stencil_hypterm_rk(const Geom &g, const double *cons, const double *q, double *flux,
+                              const double dxi[3], const double *d, const double *base, double *out,
+                              const double *dt, int phase, hipStream_t s)
+{
+    if (pipe_on({cons, q, flux, d, base, out}))
+        return launch_hypterm_pipe(g, cons, q, flux, dxi[0], dxi[1], dxi[2], d, base, out, dt, phase, s);
+    return launch_hypterm_rk(g, cons, q, flux, dxi[0], dxi[1], dxi[2], d, base, out, dt, phase, s);
+}
+
+// wz: scratch of one ghosted component (g.cg doubles), or null when the caller has
+// none (the register-staged kernel needs none).
 hipError_t stencil_diffterm(const Geom &g, const double *q, double *difflux, const double dxi[3],
-                            double eta, double alam, hipStream_t s)
+                            double eta, double alam, double *wz, hipStream_t s)
 {
     if (g_stencil_impl == CNS_IMPL_DIRECT)
         return launch_diffterm_direct(g, q, difflux, dxi[0], dxi[1], dxi[2], eta, alam, s);
+    if (wz && pipe_on({q, difflux, wz})) {
+        hipError_t e = launch_wz(g, q, wz, dxi[2], s);
+        if (e != hipSuccess) return e;
+        return launch_diffterm_pipe(g, q, wz, difflux, dxi[0], dxi[1], dxi[2], eta, alam, s);
+    }
     return launch_diffterm_tiled(g, q, difflux, dxi[0], dxi[1], dxi[2], eta, alam, s);
 }
 
@@ -169,6 +238,8 @@ struct cns_ctx {
     // fused hypterm+update never writes an array it reads with halos.  Null when the
     // unfused composition is selected.
     double *d_stage;
+    // wz field of the pipelined diffterm (one ghosted component).
+    double *d_wz;
     Profiler prof;
 };
 
@@ -264,7 +335,12 @@ int cns_diffterm(const int lo[3], const int hi[3], int ng, const double dx[3], c
     if (rc) return rc;
     if (!dx || !q || !difflux) return fail(CNS_EINVAL, "NULL argument");
     double dxi[3] = {1.0E0 / dx[0], 1.0E0 / dx[1], 1.0E0 / dx[2]};  // cpu/diffterm.cpp:54-57
-    HIPCHK(stencil_diffterm(g, q, difflux, dxi, eta, alam, t_stream));
+    double *wz = nullptr;
+    if (g_stencil_impl == CNS_IMPL_TILED && pipe_on({q, difflux})) {
+        rc = thread_scratch((size_t)g.cg, &wz);
+        if (rc) return rc;
+    }
+    HIPCHK(stencil_diffterm(g, q, difflux, dxi, eta, alam, wz, t_stream));
     return CNS_OK;
 }
 
@@ -312,12 +388,15 @@ int cns_ctx_create(cns_ctx **ctx, const int ncells[3], const double dx[3], doubl
     c->d_dt = (double *)blk + 1;
     c->d_courno = (double *)blk + 2;
     c->d_stage = nullptr;
-    if (hipMalloc((void **)&c->d_stage, (size_t)NC * c->g.cg * sizeof(double)) != hipSuccess) {
+    c->d_wz = nullptr;
+    if (hipMalloc((void **)&c->d_stage, (size_t)NC * c->g.cg * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&c->d_wz, (size_t)c->g.cg * sizeof(double)) != hipSuccess) {
         (void)hipGetLastError();
+        if (c->d_stage) (void)hipFree(c->d_stage);
         (void)hipFree(blk);
         delete c;
-        return fail(CNS_ENOMEM, "hipMalloc of the %zu-byte stage buffer failed",
-                    (size_t)NC * c->g.cg * sizeof(double));
+        return fail(CNS_ENOMEM, "hipMalloc of the %zu-byte stage and wz buffers failed",
+                    (size_t)(NC + 1) * c->g.cg * sizeof(double));
     }
     *ctx = c;
     return CNS_OK;
@@ -345,6 +424,7 @@ int cns_ctx_destroy(cns_ctx *ctx)
     if (!ctx) return CNS_OK;
     (void)hipFree(ctx->d_acc);
     if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    if (ctx->d_wz) (void)hipFree(ctx->d_wz);
     delete ctx;
     return CNS_OK;
 }
@@ -368,18 +448,18 @@ int cns_advance_async(cns_ctx *ctx, double *U, double *Unew, double *Q, double *
         HIPCHK(launch_courno_reset(ctx->d_acc, 1.0E-50, s));
         TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, U, Q, ctx->d_acc, ctx->dx[0], ctx->dx[1], ctx->dx[2], s));
         HIPCHK(launch_courno_to_dt(ctx->d_acc, ctx->cfl, dt, courno_dev ? courno_dev : ctx->d_courno, s));
-        TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, s));
-        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_rk(g, U, Q, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, T, dt, 1, s));
+        TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
+        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, stencil_hypterm_rk(g, U, Q, F, ctx->dxi, D, U, T, dt, 1, s));
 
         TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, T, s));
         TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, T, Q, nullptr, 0.0, 0.0, 0.0, s));
-        TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, s));
-        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_rk(g, T, Q, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, Unew, dt, 2, s));
+        TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
+        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, stencil_hypterm_rk(g, T, Q, F, ctx->dxi, D, U, Unew, dt, 2, s));
 
         TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, Unew, s));
         TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, Unew, Q, nullptr, 0.0, 0.0, 0.0, s));
-        TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, s));
-        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_rk(g, Unew, Q, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, U, dt, 3, s));
+        TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
+        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, stencil_hypterm_rk(g, Unew, Q, F, ctx->dxi, D, U, U, dt, 3, s));
         return CNS_OK;
     }
     // Unfused composition, kernel for kernel as the reference.
@@ -388,19 +468,19 @@ int cns_advance_async(cns_ctx *ctx, double *U, double *Unew, double *Q, double *
     HIPCHK(launch_courno_reset(ctx->d_acc, 1.0E-50, s));
     TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, U, Q, ctx->d_acc, ctx->dx[0], ctx->dx[1], ctx->dx[2], s));
     HIPCHK(launch_courno_to_dt(ctx->d_acc, ctx->cfl, dt, courno_dev ? courno_dev : ctx->d_courno, s));
-    TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, s));
+    TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
     TIMED(ctx, CNS_PROF_HYPTERM, s, stencil_hypterm(g, U, Q, F, ctx->dxi, s));
     TIMED(ctx, CNS_PROF_RK, s, launch_rk_update(g, 1, U, Unew, D, F, dt, 0.0, s));
     // :90-119: stage 2
     TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, Unew, s));
     TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, Unew, Q, nullptr, 0.0, 0.0, 0.0, s));
-    TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, s));
+    TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
     TIMED(ctx, CNS_PROF_HYPTERM, s, stencil_hypterm(g, Unew, Q, F, ctx->dxi, s));
     TIMED(ctx, CNS_PROF_RK, s, launch_rk_update(g, 2, U, Unew, D, F, dt, 0.0, s));
     // :124-153: stage 3
     TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, Unew, s));
     TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, Unew, Q, nullptr, 0.0, 0.0, 0.0, s));
-    TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, s));
+    TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
     TIMED(ctx, CNS_PROF_HYPTERM, s, stencil_hypterm(g, Unew, Q, F, ctx->dxi, s));
     TIMED(ctx, CNS_PROF_RK, s, launch_rk_update(g, 3, U, Unew, D, F, dt, 0.0, s));
     return CNS_OK;

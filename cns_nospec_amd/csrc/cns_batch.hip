@@ -280,12 +280,14 @@ __device__ __forceinline__ void diff_tile(BoxDev *__restrict__ boxes, int b, int
         wvy[7] = vym;
     }
 
+    FT nu, nv, nw, nt;  // prefetched own values of the next leading plane
+    {
+        const int o = wrapc(z0 + 4, e) * e2 + own;
+        nu = q_u[o]; nv = q_v[o]; nw = q_w[o]; nt = q_t[o];
+    }
     for (int k = z0; k < z0 + nz; k++) {
         const int cp = k * e2, lp = wrapc(k + 4, e) * e2;
-        {
-            const int o = lp + own;
-            wu[8] = q_u[o]; wv[8] = q_v[o]; ww[8] = q_w[o]; wt[8] = q_t[o];
-        }
+        wu[8] = nu; wv[8] = nv; ww[8] = nw; wt[8] = nt;
         // phase A: lead-plane u (x halo), v (y halo); centre u, v (square), w, T (cross)
         __syncthreads();
         s_lu[me] = wu[8]; s_lv[me] = wv[8];
@@ -342,6 +344,10 @@ __device__ __forceinline__ void diff_tile(BoxDev *__restrict__ boxes, int b, int
 #undef GWZ
         }
         __syncthreads();
+        {   // next leading plane; its latency hides behind phase C
+            const int o = wrapc(k + 5, e) * e2 + own;
+            nu = q_u[o]; nv = q_v[o]; nw = q_w[o]; nt = q_t[o];
+        }
         // phase C (cpu/diffterm.cpp:155-311)
         if (t.active) {
 #define CUx(m) s_u[me + (m)]
@@ -445,16 +451,23 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
         const int o = wrapc(z0 - 4 + m, e) * e2 + own;
         ww[m] = q_w[o]; wp[m] = q_p[o];
     }
+    // prefetched: next leading plane (w, p) and next centre (u, v, conserved)
+    FT nw, np, nu, nv;
+    double nmx, nmy, nmz, nE;
+    {
+        const int o = wrapc(z0 + 4, e) * e2 + own, oc = z0 * e2 + own;
+        nw = q_w[o]; np = q_p[o];
+        nu = q_u[oc]; nv = q_v[oc];
+        nmx = c_mx[oc]; nmy = c_my[oc]; nmz = c_mz[oc]; nE = c_E[oc];
+    }
     unsigned long long cmax = 0ull;
     for (int k = z0; k < z0 + nz; k++) {
         const int cp = k * e2;
         const int zo = cp + own;
-        {
-            const int o = wrapc(k + 4, e) * e2 + own;
-            ww[8] = q_w[o]; wp[8] = q_p[o];
-        }
-        const FT u_own = q_u[zo], v_own = q_v[zo];
-        const FT mx_own = (FT)c_mx[zo], my_own = (FT)c_my[zo], mz_own = (FT)c_mz[zo], E_own = (FT)c_E[zo];
+        ww[8] = nw; wp[8] = np;
+        const FT u_own = nu, v_own = nv;
+        const FT mx_own = (FT)nmx, my_own = (FT)nmy, mz_own = (FT)nmz, E_own = (FT)nE;
+        const double cur_c[5] = {0.0, nmx, nmy, nmz, nE};
         __syncthreads();
         s_mx[me] = mx_own; s_my[me] = my_own; s_mz[me] = mz_own; s_E[me] = E_own; s_p[me] = wp[4];
         s_u[me] = u_own; s_v[me] = v_own;
@@ -473,6 +486,20 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
             s_p[l] = q_p[o]; s_v[l] = q_v[o];
         }
         __syncthreads();
+        if (k + 1 < z0 + nz) {  // next iteration's operands, hidden behind the compute
+            const int o = wrapc(k + 5, e) * e2 + own, oc = zo + e2;
+            nw = q_w[o]; np = q_p[o];
+            nu = q_u[oc]; nv = q_v[oc];
+            nmx = c_mx[oc]; nmy = c_my[oc]; nmz = c_mz[oc]; nE = c_E[oc];
+        }
+        double rkd[4], rkb[5], rho_c = 0.0;
+        if (t.active) {  // this cell's RK operands
+            rho_c = c_r[zo];
+#pragma unroll
+            for (int l = 0; l < 4; l++) rkd[l] = (double)din[l * e3 + zo];
+#pragma unroll
+            for (int l = 0; l < NC; l++) rkb[l] = b_0[l * e3 + zo];
+        }
         if (t.active) {
             FT F[5];
 #define XMX(m) s_mx[me + (m)]
@@ -521,9 +548,9 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
             double o[5];
 #pragma unroll
             for (int l = 0; l < NC; l++) {
-                const double dl = l == 0 ? 0.0E0 : (double)din[(l - 1) * e3 + zo];
+                const double dl = l == 0 ? 0.0E0 : rkd[l - 1];
                 const double df = dl + (double)F[l];
-                o[l] = ra * b_0[l * e3 + zo] + rb * (c_r[l * e3 + zo] + dt * df);
+                o[l] = ra * rkb[l] + rb * ((l == 0 ? rho_c : cur_c[l]) + dt * df);
                 o_0[l * e3 + zo] = o[l];
             }
             double u, v, w, p, T;

@@ -99,7 +99,7 @@ struct RkArgs {
     int phase;
 };
 
-template <int BY, int WPE, int ZWIN, bool RK>
+template <int BY, int WPE, bool RK>
 __global__ void __launch_bounds__(TX *BY, WPE)
 hypterm_kernel(Geom g, const double *__restrict__ cons, const double *__restrict__ q,
                double *__restrict__ flux, double dxi0, double dxi1, double dxi2, int kz, int ntx,
@@ -124,43 +124,42 @@ hypterm_kernel(Geom g, const double *__restrict__ cons, const double *__restrict
     const double *q_u = q + QU * cg, *q_v = q + QV * cg, *q_w = q + QW * cg, *q_p = q + QPRES * cg;
     const int64_t own = (int64_t)gj * g.xg + gi;
     const int me = (ty + 4) * PX + tx + 4;
-    double dt = 0.0;
-    if (RK) dt = rk.dt[0];
+    double dt = 0.0, ra = 0.0, rb = 1.0;
+    if (RK) {
+        // one branch-free form for the three stages: r = ra*base + rb*(cons + dt*(D+F));
+        // (ra, rb) = (0, 1), (3/4, 1/4), (1/3, 2/3); 0*base + 1*x is exactly x, so
+        // every phase keeps the reference's rounding (cpu/advance.cpp:82, :114-116, :148-150)
+        dt = rk.dt[0];
+        ra = rk.phase == 1 ? 0.0E0 : (rk.phase == 2 ? ThreeQuarters : OneThird);
+        rb = rk.phase == 1 ? 1.0E0 : (rk.phase == 2 ? OneQuarter : TwoThirds);
+    }
 
+    // register windows (plane k+m at [4+m]) and the prefetched next leading plane
     double wmx[9], wmy[9], wmz[9], wE[9], ww[9], wp[9];
-#define LOADW(m, o)                                \
-    do {                                           \
-        if (ZWIN & 1) wmx[m] = c_mx[o];            \
-        if (ZWIN & 2) wmy[m] = c_my[o];            \
-        if (ZWIN & 4) wmz[m] = c_mz[o];            \
-        if (ZWIN & 8) wE[m] = c_E[o];              \
-        if (ZWIN & 16) ww[m] = q_w[o];             \
-        if (ZWIN & 32) wp[m] = q_p[o];             \
-    } while (0)
-    if (ZWIN && inbox) {
+    double nmx = 0, nmy = 0, nmz = 0, nE = 0, nw = 0, np = 0, nu = 0, nv = 0;
+    if (inbox) {
 #pragma unroll
-        for (int m = 0; m < 8; m++) LOADW(m, (int64_t)(kbeg + m) * pg + own);
+        for (int m = 0; m < 8; m++) {
+            const int64_t o = (int64_t)(kbeg + m) * pg + own;
+            wmx[m] = c_mx[o]; wmy[m] = c_my[o]; wmz[m] = c_mz[o];
+            wE[m] = c_E[o]; ww[m] = q_w[o]; wp[m] = q_p[o];
+        }
+        const int64_t o = (int64_t)(kbeg + 2 * NG) * pg + own;
+        nmx = c_mx[o]; nmy = c_my[o]; nmz = c_mz[o]; nE = c_E[o]; nw = q_w[o]; np = q_p[o];
+        const int64_t oc = (int64_t)(kbeg + NG) * pg + own;
+        nu = q_u[oc];
+        nv = q_v[oc];
     }
 
     for (int k = kbeg; k < kend; k++) {
         const int64_t plane = (int64_t)(k + NG) * pg;
         const int64_t zo = plane + own;  // own cell of the centre plane
-        double c_own[6] = {0, 0, 0, 0, 0, 0};  // mx my mz E p at the centre, u, v
-        double u_own = 0.0, v_own = 0.0;
-        if (inbox) {
-            if (ZWIN) LOADW(8, zo + 4 * pg);
-            c_own[0] = (ZWIN & 1) ? wmx[4] : c_mx[zo];
-            c_own[1] = (ZWIN & 2) ? wmy[4] : c_my[zo];
-            c_own[2] = (ZWIN & 4) ? wmz[4] : c_mz[zo];
-            c_own[3] = (ZWIN & 8) ? wE[4] : c_E[zo];
-            c_own[4] = (ZWIN & 32) ? wp[4] : q_p[zo];
-            u_own = q_u[zo];
-            v_own = q_v[zo];
-        }
+        wmx[8] = nmx; wmy[8] = nmy; wmz[8] = nmz; wE[8] = nE; ww[8] = nw; wp[8] = np;
+        const double u_own = nu, v_own = nv;
         __syncthreads();
         if (inbox) {
-            s_mx[me] = c_own[0]; s_my[me] = c_own[1]; s_mz[me] = c_own[2];
-            s_E[me] = c_own[3]; s_p[me] = c_own[4]; s_u[me] = u_own; s_v[me] = v_own;
+            s_mx[me] = wmx[4]; s_my[me] = wmy[4]; s_mz[me] = wmz[4];
+            s_E[me] = wE[4]; s_p[me] = wp[4]; s_u[me] = u_own; s_v[me] = v_own;
         }
         for (int h = threadIdx.x; h < 8 * BY; h += NT) {
             int r, c;
@@ -185,6 +184,24 @@ hypterm_kernel(Geom g, const double *__restrict__ cons, const double *__restrict
             }
         }
         __syncthreads();
+        // Loads whose latency the compute below hides: the leading plane of the next
+        // iteration, the next centre u and v, and this cell's RK operands.
+        if (inbox && k + 1 < kend) {
+            const int64_t o = zo + 5 * pg;
+            nmx = c_mx[o]; nmy = c_my[o]; nmz = c_mz[o]; nE = c_E[o]; nw = q_w[o]; np = q_p[o];
+            nu = q_u[zo + pg];
+            nv = q_v[zo + pg];
+        }
+        const int64_t fo = iidx(g, gi - NG, gj - NG, k);
+        double rkd[5], rkb[5], rho_c = 0.0;
+        if (RK && active) {
+            rho_c = cons[zo];
+#pragma unroll
+            for (int l = 0; l < NC; l++) {
+                rkd[l] = rk.d[l * g.c + fo];
+                rkb[l] = rk.base[l * cg + zo];
+            }
+        }
 
         if (active) {
             double F[5];
@@ -212,51 +229,33 @@ hypterm_kernel(Geom g, const double *__restrict__ cons, const double *__restrict
             F[IMY] = F[IMY] - (HMOMP(YMY, YV, YP)) * dxi1;
             F[IMZ] = F[IMZ] - (HMOM(YMZ, YV)) * dxi1;
             F[IENE] = F[IENE] - (HENE(YE, YV, YP)) * dxi1;
-            // ---- z (cpu/hypterm.cpp:158-204) from the window or global memory
-#define ZMX(m) ((ZWIN & 1) ? wmx[4 + (m)] : c_mx[zo + (int64_t)(m) * pg])
-#define ZMY(m) ((ZWIN & 2) ? wmy[4 + (m)] : c_my[zo + (int64_t)(m) * pg])
-#define ZMZ(m) ((ZWIN & 4) ? wmz[4 + (m)] : c_mz[zo + (int64_t)(m) * pg])
-#define ZE(m) ((ZWIN & 8) ? wE[4 + (m)] : c_E[zo + (int64_t)(m) * pg])
-#define ZW(m) ((ZWIN & 16) ? ww[4 + (m)] : q_w[zo + (int64_t)(m) * pg])
-#define ZP(m) ((ZWIN & 32) ? wp[4 + (m)] : q_p[zo + (int64_t)(m) * pg])
+            // ---- z (cpu/hypterm.cpp:158-204) from the register windows
+#define ZMX(m) wmx[4 + (m)]
+#define ZMY(m) wmy[4 + (m)]
+#define ZMZ(m) wmz[4 + (m)]
+#define ZE(m) wE[4 + (m)]
+#define ZW(m) ww[4 + (m)]
+#define ZP(m) wp[4 + (m)]
             F[IRHO] = F[IRHO] - (D1(ZMZ)) * dxi2;
             F[IMX] = F[IMX] - (HMOM(ZMX, ZW)) * dxi2;
             F[IMY] = F[IMY] - (HMOM(ZMY, ZW)) * dxi2;
             F[IMZ] = F[IMZ] - (HMOMP(ZMZ, ZW, ZP)) * dxi2;
             F[IENE] = F[IENE] - (HENE(ZE, ZW, ZP)) * dxi2;
-            const int64_t fo = iidx(g, gi - NG, gj - NG, k);
 #pragma unroll
             for (int l = 0; l < NC; l++) flux[l * g.c + fo] = F[l];
             if (RK) {
-                // cpu/advance.cpp:82 (phase 1), :114-116 (phase 2), :148-150 (phase 3)
-                const double uc[5] = {cons[zo], c_own[0], c_own[1], c_own[2], c_own[3]};
+                const double uc[5] = {rho_c, wmx[4], wmy[4], wmz[4], wE[4]};
 #pragma unroll
-                for (int l = 0; l < NC; l++) {
-                    const double df = rk.d[l * g.c + fo] + F[l];
-                    double r;
-                    if (rk.phase == 1)
-                        r = uc[l] + dt * df;
-                    else if (rk.phase == 2)
-                        r = ThreeQuarters * rk.base[l * cg + zo] + OneQuarter * (uc[l] + dt * df);
-                    else
-                        r = OneThird * rk.base[l * cg + zo] + TwoThirds * (uc[l] + dt * df);
-                    rk.out[l * cg + zo] = r;
-                }
+                for (int l = 0; l < NC; l++)
+                    rk.out[l * cg + zo] = ra * rkb[l] + rb * (uc[l] + dt * (rkd[l] + F[l]));
             }
         }
-        if (ZWIN) {
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                if (ZWIN & 1) wmx[m] = wmx[m + 1];
-                if (ZWIN & 2) wmy[m] = wmy[m + 1];
-                if (ZWIN & 4) wmz[m] = wmz[m + 1];
-                if (ZWIN & 8) wE[m] = wE[m + 1];
-                if (ZWIN & 16) ww[m] = ww[m + 1];
-                if (ZWIN & 32) wp[m] = wp[m + 1];
-            }
+        for (int m = 0; m < 8; m++) {
+            wmx[m] = wmx[m + 1]; wmy[m] = wmy[m + 1]; wmz[m] = wmz[m + 1];
+            wE[m] = wE[m + 1]; ww[m] = ww[m + 1]; wp[m] = wp[m + 1];
         }
     }
-#undef LOADW
 }
 
 // ====================================================================== diffterm
@@ -365,19 +364,23 @@ diffterm_kernel(Geom g, const double *__restrict__ q, double *__restrict__ diffl
         }
     }
 
+    // prefetched own values of the next leading plane
+    double nu = 0.0, nv = 0.0, nw = 0.0, nt = 0.0;
+    if (inbox) {
+        const int64_t o = (int64_t)(kbeg + 2 * NG) * pg + own;
+        nu = q_u[o]; nv = q_v[o]; nw = q_w[o]; nt = q_t[o];
+    }
     for (int k = kbeg; k < kend; k++) {
         const int64_t cplane = (int64_t)(k + NG) * pg;
         const int64_t lplane = (int64_t)(k + 2 * NG) * pg;
         const int64_t zo = cplane + own;
-        double lead_u = 0.0, lead_v = 0.0, cen[4] = {0, 0, 0, 0};
+        const double lead_u = nu, lead_v = nv;
+        double cen[4] = {0, 0, 0, 0};
         if (inbox) {
-            const int64_t o = lplane + own;
-            lead_u = q_u[o];
-            lead_v = q_v[o];
-            if (ZWIN & 1) wu[8] = lead_u;
-            if (ZWIN & 2) wv[8] = lead_v;
-            if (ZWIN & 4) ww[8] = q_w[o];
-            if (ZWIN & 8) wt[8] = q_t[o];
+            if (ZWIN & 1) wu[8] = nu;
+            if (ZWIN & 2) wv[8] = nv;
+            if (ZWIN & 4) ww[8] = nw;
+            if (ZWIN & 8) wt[8] = nt;
             cen[0] = (ZWIN & 1) ? wu[4] : q_u[zo];
             cen[1] = (ZWIN & 2) ? wv[4] : q_v[zo];
             cen[2] = (ZWIN & 4) ? ww[4] : q_w[zo];
@@ -469,6 +472,11 @@ diffterm_kernel(Geom g, const double *__restrict__ q, double *__restrict__ diffl
             }
         }
         __syncthreads();
+        // the next iteration's leading plane: its latency hides behind phase C
+        if (inbox && k + 1 < kend) {
+            const int64_t o = lplane + pg + own;
+            nu = q_u[o]; nv = q_v[o]; nw = q_w[o]; nt = q_t[o];
+        }
 
         // ---------------- phase C
         if (active) {
@@ -587,14 +595,14 @@ int pick_kz(const Geom &g, int tiles_xy)
     return kz;
 }
 
-template <int BY, int WPE, int ZWIN, bool RK>
+template <int BY, int WPE, bool RK>
 hipError_t run_hyp(const Geom &g, const double *cons, const double *q, double *flux, double dxi0,
                    double dxi1, double dxi2, const RkArgs &rk, hipStream_t s)
 {
     const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
     const int kz = pick_kz(g, ntx * nty);
     const int ntz = (g.n2 + kz - 1) / kz;
-    hypterm_kernel<BY, WPE, ZWIN, RK><<<ntx * nty * ntz, TX * BY, 0, s>>>(
+    hypterm_kernel<BY, WPE, RK><<<ntx * nty * ntz, TX * BY, 0, s>>>(
         g, cons, q, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk);
     return hipGetLastError();
 }
@@ -622,13 +630,10 @@ hipError_t hyp_dispatch(const Geom &g, const double *cons, const double *q, doub
                         double dxi0, double dxi1, double dxi2, const RkArgs &rk, hipStream_t s)
 {
     switch (env_variant("CNS_HYP_VARIANT")) {
-    case 1: return run_hyp<4, 2, 63, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
-    case 2: return run_hyp<4, 3, 48, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
-    case 3: return run_hyp<4, 4, 0, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
-    case 4: return run_hyp<8, 2, 48, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
-    case 5: return run_hyp<8, 2, 0, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
-    case 6: return run_hyp<4, 3, 0, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
-    default: return run_hyp<4, 1, 63, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    case 1: return run_hyp<4, 3, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    case 2: return run_hyp<8, 2, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    case 3: return run_hyp<4, 1, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
+    default: return run_hyp<4, 2, RK>(g, cons, q, flux, dxi0, dxi1, dxi2, rk, s);
     }
 }
 
@@ -654,12 +659,10 @@ hipError_t launch_diffterm_tiled(const Geom &g, const double *q, double *difflux
                                  double dxi1, double dxi2, double eta, double alam, hipStream_t s)
 {
     switch (env_variant("CNS_DIFF_VARIANT")) {
-    case 1: return run_diff<4, 2, 7>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
-    case 2: return run_diff<4, 2, 0>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
-    case 3: return run_diff<4, 3, 0>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
-    case 4: return run_diff<8, 2, 0>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
-    case 5: return run_diff<4, 2, 15>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
-    default: return run_diff<4, 1, 15>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
+    case 1: return run_diff<4, 1, 15>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
+    case 2: return run_diff<8, 2, 15>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
+    case 3: return run_diff<4, 3, 15>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
+    default: return run_diff<4, 2, 15>(g, q, difflux, dxi0, dxi1, dxi2, eta, alam, s);
     }
 }
 

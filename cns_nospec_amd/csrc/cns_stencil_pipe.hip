@@ -1,0 +1,560 @@
+// cns_stencil_pipe.hip -- pipelined z-marching hypterm and diffterm for gfx950.
+//
+// A 512-thread workgroup owns a 64 x 8 column tile (one wave per row) and marches
+// up a run of planes.  Every field whose x/y neighbours a plane needs is staged in
+// LDS as a full 16 x 72 tile (8 rows / 4 columns of halo on each side), loaded with
+// direct-to-LDS DMA (global_load_lds_dwordx4: each lane moves 16 bytes straight
+// into the tile, no registers involved) into one of two LDS buffers.  The loads of
+// plane k+1 are issued right after the barrier that opens plane k, so the whole
+// compute of plane k hides their latency; a plane costs one barrier (hypterm) or
+// two (diffterm), and there is no per-plane gather loop.  z neighbours of the
+// lane's own column come from 9-plane register windows whose leading values are
+// prefetched one plane ahead.
+//
+// diffterm reads wz = d(w)/dz on the cross halo as a precomputed field (wz_kernel
+// below, the reference's wz temporary of cpu/diffterm.cpp:127-151 over the ghosted
+// x-y extent), so no plane needs the 8 z neighbours of halo columns.
+//
+// Arithmetic and its order are the reference's (cpu/hypterm.cpp:11-209,
+// cpu/diffterm.cpp:22-320, cpu/advance.cpp:78-153); with -ffp-contract=off the
+// results are bit-identical to the reference cpu/ path and to the other kernels.
+#include "cns_kernels.h"
+
+namespace cns {
+
+namespace {
+
+constexpr int TX = 64;              // tile width (one wave)
+constexpr int PX = TX + 8;          // LDS pitch (x halos)
+constexpr int BY = 8;               // tile height (waves per workgroup)
+constexpr int R = BY + 8;           // LDS rows (y halos)
+constexpr int NT = TX * BY;         // 512 threads
+constexpr int TILE = R * PX;        // 1152 doubles per staged field
+constexpr int TILE_DMA = TILE / 128;  // 9 wave-wide dwordx4 loads per field
+static_assert(TILE % 128 == 0 && TILE_DMA == 9, "tile must be a whole number of 1 KB DMA blocks");
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+#define D1(F) (ALP * (F(1) - F(-1)) + BET * (F(2) - F(-2)) + GAM * (F(3) - F(-3)) + DEL * (F(4) - F(-4)))
+#define D2(F)                                                                                \
+    (CENTER * F(0) + OFF1 * (F(1) + F(-1)) + OFF2 * (F(2) + F(-2)) + OFF3 * (F(3) + F(-3)) \
+     + OFF4 * (F(4) + F(-4)))
+#define HMOM(C, UN) \
+    (ALP * (C(1) * UN(1) - C(-1) * UN(-1)) + BET * (C(2) * UN(2) - C(-2) * UN(-2)) \
+     + GAM * (C(3) * UN(3) - C(-3) * UN(-3)) + DEL * (C(4) * UN(4) - C(-4) * UN(-4)))
+#define HMOMP(C, UN, P)                                                                  \
+    (ALP * (C(1) * UN(1) - C(-1) * UN(-1) + (P(1) - P(-1)))                              \
+     + BET * (C(2) * UN(2) - C(-2) * UN(-2) + (P(2) - P(-2)))                            \
+     + GAM * (C(3) * UN(3) - C(-3) * UN(-3) + (P(3) - P(-3)))                            \
+     + DEL * (C(4) * UN(4) - C(-4) * UN(-4) + (P(4) - P(-4))))
+#define HENE(C, UN, P)                                                                   \
+    (ALP * (C(1) * UN(1) - C(-1) * UN(-1) + (P(1) * UN(1) - P(-1) * UN(-1)))             \
+     + BET * (C(2) * UN(2) - C(-2) * UN(-2) + (P(2) * UN(2) - P(-2) * UN(-2)))           \
+     + GAM * (C(3) * UN(3) - C(-3) * UN(-3) + (P(3) * UN(3) - P(-3) * UN(-3)))           \
+     + DEL * (C(4) * UN(4) - C(-4) * UN(-4) + (P(4) * UN(4) - P(-4) * UN(-4))))
+
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int xcd = b & 7, local = b >> 3;
+    const int per = nb >> 3, rem = nb & 7;
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// Tile geometry shared by the DMA issue: the ghosted-array coordinates of tile cell
+// (0, 0) and the clamps that keep every lane's 16-byte load inside the array (cells
+// beyond the box are only read by lanes whose outputs are discarded).
+struct TileDma {
+    int gi0, gj0, g1m1, xgm2;
+    unsigned xg;
+};
+
+// DMA block n (0..8) of one field tile: lane L moves tile elements e, e+1 with
+// e = 128 n + 2 L (row e / 72, column e % 72; 72 is even, so a pair never straddles
+// rows) from the plane at `src` into tile + 128 n + 2 L.
+__device__ __forceinline__ void dma_block(const double *src, double *tile, int n, const TileDma &t,
+                                          int lane)
+{
+    const int e = 128 * n + 2 * lane;
+    const int r = e / PX, c = e - r * PX;
+    const int hj = min(t.gj0 + r, t.g1m1), hi = min(t.gi0 + c, t.xgm2);
+    // a plane offset fits 32 bits; keeping it 32-bit halves what the compiler holds
+    // live across the plane loop
+    const unsigned off = (unsigned)hj * t.xg + (unsigned)hi;
+    __builtin_amdgcn_global_load_lds((const void *)(src + off), (lds_void *)(tile + 128 * n), 16, 0, 0);
+}
+
+// This wave's share of the 9*NF DMA blocks of NF field tiles of one plane: block
+// p = 9 f + n goes to wave p % 8, which balances 8 or 7 blocks per wave for NF = 7.
+template <int NF>
+__device__ __forceinline__ void dma_plane(const double *const (&src)[NF], int64_t plane,
+                                          double *buf, const TileDma &t, int wave, int lane)
+{
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const int n0 = (wave - f) & 7;
+        dma_block(src[f] + plane, buf + f * TILE, n0, t, lane);
+        if (n0 == 0) dma_block(src[f] + plane, buf + f * TILE, 8, t, lane);
+    }
+}
+
+// All of this wave's loads, DMA included, have landed; then the workgroup barrier.
+__device__ __forceinline__ void plane_barrier()
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
+struct PipeRk {
+    const double *d;     // difflux (interior)
+    const double *base;  // U^n (ghosted), phases 2 and 3
+    double *out;         // result (ghosted); must not alias cons
+    const double *dt;    // device dt
+    int phase;           // 0: plain hypterm
+};
+
+// ======================================================================= hypterm
+// LDS fields (centre plane): mx, my, mz, E, p, u, v.  Register windows: mx, my, mz,
+// E, w, p.  With RK the stage update of the cell is applied as in cns_stencil.hip.
+enum { HMX = 0, HMY, HMZ, HE, HP, HU, HV, HNF };
+
+template <bool RK>
+__global__ void __launch_bounds__(NT, 2)
+hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__restrict__ q,
+                    double *__restrict__ flux, double dxi0, double dxi1, double dxi2, int kz,
+                    int ntx, int nty, PipeRk rk)
+{
+    __shared__ double sm[2 * HNF * TILE];
+
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const TileDma td{bx * TX, by * BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
+    const int gi = td.gi0 + lane + 4, gj = td.gj0 + wave + 4;
+    const bool inbox = gi < g.g0 && gj < g.g1;
+    const bool active = gi < g.n0 + NG && gj < g.n1 + NG;
+    const int kbeg = bz * kz, kend = min(kbeg + kz, g.n2);
+    const int64_t cg = g.cg, pg = g.pg;
+    const double *c_mx = cons + IMX * cg, *c_my = cons + IMY * cg, *c_mz = cons + IMZ * cg,
+                 *c_E = cons + IENE * cg;
+    const double *q_u = q + QU * cg, *q_v = q + QV * cg, *q_w = q + QW * cg, *q_p = q + QPRES * cg;
+    const double *const src[HNF] = {c_mx, c_my, c_mz, c_E, q_p, q_u, q_v};
+    const int64_t own = (int64_t)gj * g.xg + gi;
+    const int me = (wave + 4) * PX + lane + 4;
+    double dt = 0.0, ra = 0.0, rb = 1.0;
+    if (RK) {
+        // r = ra*base + rb*(cons + dt*(D+F)) with (ra, rb) = (0, 1), (3/4, 1/4), (1/3, 2/3):
+        // 0*base + 1*x is exactly x, so each phase keeps the reference's rounding
+        // (cpu/advance.cpp:82, :114-116, :148-150)
+        dt = rk.dt[0];
+        ra = rk.phase == 1 ? 0.0E0 : (rk.phase == 2 ? ThreeQuarters : OneThird);
+        rb = rk.phase == 1 ? 1.0E0 : (rk.phase == 2 ? OneQuarter : TwoThirds);
+    }
+
+    // first plane's tiles in flight while the windows fill
+    dma_plane<HNF>(src, (int64_t)(kbeg + NG) * pg, sm, td, wave, lane);
+
+    double wmx[9], wmy[9], wmz[9], wE[9], ww[9], wp[9];
+    double nmx = 0, nmy = 0, nmz = 0, nE = 0, nw = 0, np = 0;
+    if (inbox) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int64_t o = (int64_t)(kbeg + m) * pg + own;
+            wmx[m] = c_mx[o]; wmy[m] = c_my[o]; wmz[m] = c_mz[o];
+            wE[m] = c_E[o]; ww[m] = q_w[o]; wp[m] = q_p[o];
+        }
+        const int64_t o = (int64_t)(kbeg + 2 * NG) * pg + own;
+        nmx = c_mx[o]; nmy = c_my[o]; nmz = c_mz[o]; nE = c_E[o]; nw = q_w[o]; np = q_p[o];
+    }
+
+    for (int k = kbeg; k < kend; k++) {
+        const int cur = (k - kbeg) & 1;
+        const double *b = sm + cur * (HNF * TILE);
+        plane_barrier();
+        if (k + 1 < kend) dma_plane<HNF>(src, (int64_t)(k + 1 + NG) * pg, sm + (cur ^ 1) * (HNF * TILE), td, wave, lane);
+
+        const int64_t zo = (int64_t)(k + NG) * pg + own;
+        wmx[8] = nmx; wmy[8] = nmy; wmz[8] = nmz; wE[8] = nE; ww[8] = nw; wp[8] = np;
+        if (inbox && k + 1 < kend) {
+            const int64_t o = zo + 5 * pg;
+            nmx = c_mx[o]; nmy = c_my[o]; nmz = c_mz[o]; nE = c_E[o]; nw = q_w[o]; np = q_p[o];
+        }
+        const int64_t fo = iidx(g, gi - NG, gj - NG, k);
+        double rkd[5], rkb[5], rho_c = 0.0;
+        if (RK && active) {
+            rho_c = cons[zo];
+#pragma unroll
+            for (int l = 0; l < NC; l++) {
+                rkd[l] = rk.d[l * g.c + fo];
+                rkb[l] = rk.base[l * cg + zo];
+            }
+        }
+
+        if (active) {
+            const double *s_mx = b + HMX * TILE, *s_my = b + HMY * TILE, *s_mz = b + HMZ * TILE,
+                         *s_E = b + HE * TILE, *s_p = b + HP * TILE, *s_u = b + HU * TILE,
+                         *s_v = b + HV * TILE;
+            double F[5];
+            // ---- x (cpu/hypterm.cpp:33-90): F = -(sum)*dxinv(1)
+#define XMX(m) s_mx[me + (m)]
+#define XMY(m) s_my[me + (m)]
+#define XMZ(m) s_mz[me + (m)]
+#define XE(m) s_E[me + (m)]
+#define XP(m) s_p[me + (m)]
+#define XU(m) s_u[me + (m)]
+            F[IRHO] = -(D1(XMX)) * dxi0;
+            F[IMX] = -(HMOMP(XMX, XU, XP)) * dxi0;
+            F[IMY] = -(HMOM(XMY, XU)) * dxi0;
+            F[IMZ] = -(HMOM(XMZ, XU)) * dxi0;
+            F[IENE] = -(HENE(XE, XU, XP)) * dxi0;
+            // ---- y (cpu/hypterm.cpp:101-147): F = F - (sum)*dxinv(2)
+#define YMX(m) s_mx[me + (m) * PX]
+#define YMY(m) s_my[me + (m) * PX]
+#define YMZ(m) s_mz[me + (m) * PX]
+#define YE(m) s_E[me + (m) * PX]
+#define YP(m) s_p[me + (m) * PX]
+#define YV(m) s_v[me + (m) * PX]
+            F[IRHO] = F[IRHO] - (D1(YMY)) * dxi1;
+            F[IMX] = F[IMX] - (HMOM(YMX, YV)) * dxi1;
+            F[IMY] = F[IMY] - (HMOMP(YMY, YV, YP)) * dxi1;
+            F[IMZ] = F[IMZ] - (HMOM(YMZ, YV)) * dxi1;
+            F[IENE] = F[IENE] - (HENE(YE, YV, YP)) * dxi1;
+            // ---- z (cpu/hypterm.cpp:158-204) from the register windows
+#define ZMX(m) wmx[4 + (m)]
+#define ZMY(m) wmy[4 + (m)]
+#define ZMZ(m) wmz[4 + (m)]
+#define ZE(m) wE[4 + (m)]
+#define ZW(m) ww[4 + (m)]
+#define ZP(m) wp[4 + (m)]
+            F[IRHO] = F[IRHO] - (D1(ZMZ)) * dxi2;
+            F[IMX] = F[IMX] - (HMOM(ZMX, ZW)) * dxi2;
+            F[IMY] = F[IMY] - (HMOM(ZMY, ZW)) * dxi2;
+            F[IMZ] = F[IMZ] - (HMOMP(ZMZ, ZW, ZP)) * dxi2;
+            F[IENE] = F[IENE] - (HENE(ZE, ZW, ZP)) * dxi2;
+#undef XMX
+#undef XMY
+#undef XMZ
+#undef XE
+#undef XP
+#undef XU
+#undef YMX
+#undef YMY
+#undef YMZ
+#undef YE
+#undef YP
+#undef YV
+#undef ZMX
+#undef ZMY
+#undef ZMZ
+#undef ZE
+#undef ZW
+#undef ZP
+#pragma unroll
+            for (int l = 0; l < NC; l++) flux[l * g.c + fo] = F[l];
+            if (RK) {
+                const double uc[5] = {rho_c, wmx[4], wmy[4], wmz[4], wE[4]};
+#pragma unroll
+                for (int l = 0; l < NC; l++)
+                    rk.out[l * cg + zo] = ra * rkb[l] + rb * (uc[l] + dt * (rkd[l] + F[l]));
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            wmx[m] = wmx[m + 1]; wmy[m] = wmy[m + 1]; wmz[m] = wmz[m + 1];
+            wE[m] = wE[m + 1]; ww[m] = ww[m + 1]; wp[m] = wp[m + 1];
+        }
+    }
+}
+
+// ============================================================================ wz
+// wz(i, j, k) = D1_z(w) * dxinv(3) for every (i, j) of the ghosted x-y extent and
+// interior k (cpu/diffterm.cpp:126-151), into a one-component array with the ghosted
+// layout (plane k+ng).  One thread per column and run of kz planes, 9-plane window.
+__global__ void __launch_bounds__(256)
+wz_kernel(Geom g, const double *__restrict__ q_w, double *__restrict__ wz, double dxi2, int kz)
+{
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;  // j*xg + i
+    if (p >= g.pg || (int)(p % g.xg) >= g.g0) return;
+    const int kbeg = blockIdx.y * kz, kend = min(kbeg + kz, g.n2);
+    const int64_t pg = g.pg;
+    double w[9];
+#pragma unroll
+    for (int m = 0; m < 8; m++) w[m] = q_w[(int64_t)(kbeg + m) * pg + p];
+    for (int k = kbeg; k < kend; k++) {
+        w[8] = q_w[(int64_t)(k + 2 * NG) * pg + p];
+#define WZ(mm) w[4 + (mm)]
+        wz[(int64_t)(k + NG) * pg + p] = D1(WZ) * dxi2;
+#undef WZ
+#pragma unroll
+        for (int m = 0; m < 8; m++) w[m] = w[m + 1];
+    }
+}
+
+// ====================================================================== diffterm
+// LDS fields, double-buffered: centre plane u, v (square halos), w, T, wz (cross
+// halos), leading plane (k+4) u and v.  Single-buffered: ux on the tile and y-halo
+// rows, vy on the tile and x-halo columns.  Per plane:
+//  B  (after the opening barrier) ux, vy of the leading plane into the register
+//     windows; ux, vy of the centre plane on their halos into LDS;
+//  C  (after a second barrier) every output of cpu/diffterm.cpp:155-315 for the cell.
+// Register windows: u, v, w, T and the computed ux, vy.
+enum { DCU = 0, DCV, DCW, DCT, DWZ, DLU, DLV, DNF };
+
+__global__ void __launch_bounds__(NT, 2)
+diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restrict__ wzf,
+                     double *__restrict__ difflux, double dxi0, double dxi1, double dxi2, double eta,
+                     double alam, int kz, int ntx, int nty)
+{
+    __shared__ double sm[2 * DNF * TILE];
+    __shared__ double s_ux[R * TX];   // row r (0..15), column 4+c
+    __shared__ double s_vy[BY * PX];  // row 4+r (r 0..7), column c (0..71)
+
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const TileDma td{bx * TX, by * BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
+    const int gi = td.gi0 + lane + 4, gj = td.gj0 + wave + 4;
+    const bool inbox = gi < g.g0 && gj < g.g1;
+    const bool active = gi < g.n0 + NG && gj < g.n1 + NG;
+    const int kbeg = bz * kz, kend = min(kbeg + kz, g.n2);
+    const int64_t cg = g.cg, pg = g.pg, xg = g.xg;
+    const double *q_u = q + QU * cg, *q_v = q + QV * cg, *q_w = q + QW * cg, *q_t = q + QTEMP * cg;
+    const double *const src[DNF] = {q_u, q_v, q_w, q_t, wzf, q_u + 4 * pg, q_v + 4 * pg};
+    const int64_t own = (int64_t)gj * xg + gi;
+    const int me = (wave + 4) * PX + lane + 4;  // own cell in 16 x 72 tiles
+    const int me64 = (wave + 4) * TX + lane;    // own cell in s_ux
+    const int meby = wave * PX + lane + 4;      // own cell in s_vy
+    const double dxi0s = dxi0 * dxi0, dxi1s = dxi1 * dxi1, dxi2s = dxi2 * dxi2;
+
+    dma_plane<DNF>(src, (int64_t)(kbeg + NG) * pg, sm, td, wave, lane);
+
+    // windows: plane k+m at [4+m]; ux and vy of the first 8 planes straight from
+    // global memory (the same expressions as the per-plane path)
+    double wu[9], wv[9], ww[9], wt[9], wux[9], wvy[9];
+    double nw = 0.0, nt = 0.0;
+    if (active) {
+#pragma unroll 1
+        for (int m = 0; m < 8; m++) {
+            const int64_t o = (int64_t)(kbeg + m) * pg + own;
+#define GUX(mm) q_u[o + (mm)]
+#define GVY(mm) q_v[o + (int64_t)(mm) * xg]
+            const double ux_ = D1(GUX) * dxi0, vy_ = D1(GVY) * dxi1;
+#undef GUX
+#undef GVY
+#pragma unroll
+            for (int mm = 0; mm < 7; mm++) {
+                wux[mm] = wux[mm + 1];
+                wvy[mm] = wvy[mm + 1];
+            }
+            wux[7] = ux_;
+            wvy[7] = vy_;
+        }
+    }
+    if (inbox) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int64_t o = (int64_t)(kbeg + m) * pg + own;
+            wu[m] = q_u[o]; wv[m] = q_v[o]; ww[m] = q_w[o]; wt[m] = q_t[o];
+        }
+        const int64_t o = (int64_t)(kbeg + 2 * NG) * pg + own;
+        nw = q_w[o];
+        nt = q_t[o];
+    }
+
+    for (int k = kbeg; k < kend; k++) {
+        const int cur = (k - kbeg) & 1;
+        const double *b = sm + cur * (DNF * TILE);
+        plane_barrier();
+        if (k + 1 < kend) dma_plane<DNF>(src, (int64_t)(k + 1 + NG) * pg, sm + (cur ^ 1) * (DNF * TILE), td, wave, lane);
+        const double *s_cu = b + DCU * TILE, *s_cv = b + DCV * TILE, *s_cw = b + DCW * TILE,
+                     *s_ct = b + DCT * TILE, *s_wz = b + DWZ * TILE, *s_lu = b + DLU * TILE,
+                     *s_lv = b + DLV * TILE;
+
+        const int64_t zo = (int64_t)(k + NG) * pg + own;
+        wu[8] = s_lu[me];
+        wv[8] = s_lv[me];
+        ww[8] = nw;
+        wt[8] = nt;
+        if (inbox && k + 1 < kend) {
+            const int64_t o = zo + 5 * pg;
+            nw = q_w[o];
+            nt = q_t[o];
+        }
+
+        // ---------------- phase B
+        {
+#define LUX(mm) s_lu[me + (mm)]
+#define LVY(mm) s_lv[me + (mm) * PX]
+            // leading plane: ux, vy of the own column (cpu/diffterm.cpp:77-81, :110-114)
+            wux[8] = D1(LUX) * dxi0;
+            wvy[8] = D1(LVY) * dxi1;
+#undef LUX
+#undef LVY
+            // centre plane: ux on rows 0..15, columns 4..67 (x derivative of u)
+#pragma unroll
+            for (int h = threadIdx.x; h < R * TX; h += NT) {
+                const int l = (h >> 6) * PX + (h & 63) + 4;
+#define CUX(mm) s_cu[l + (mm)]
+                s_ux[h] = D1(CUX) * dxi0;
+#undef CUX
+            }
+            // centre plane: vy on rows 4..11, columns 0..71 (y derivative of v)
+            for (int h = threadIdx.x; h < BY * PX; h += NT) {
+                const int l = h + 4 * PX;
+#define CVY(mm) s_cv[l + (mm) * PX]
+                s_vy[h] = D1(CVY) * dxi1;
+#undef CVY
+            }
+        }
+        __syncthreads();
+
+        // ---------------- phase C
+        if (active) {
+#define XS(a, mm) a[me + (mm)]
+#define YS(a, mm) a[me + (mm) * PX]
+#define CUx(mm) XS(s_cu, mm)
+#define CUy(mm) YS(s_cu, mm)
+#define CUz(mm) wu[4 + (mm)]
+#define CVx(mm) XS(s_cv, mm)
+#define CVy(mm) YS(s_cv, mm)
+#define CVz(mm) wv[4 + (mm)]
+#define CWx(mm) XS(s_cw, mm)
+#define CWy(mm) YS(s_cw, mm)
+#define CWz(mm) ww[4 + (mm)]
+#define CTx(mm) XS(s_ct, mm)
+#define CTy(mm) YS(s_ct, mm)
+#define CTz(mm) wt[4 + (mm)]
+#define VYx(mm) s_vy[meby + (mm)]
+#define WZx(mm) XS(s_wz, mm)
+#define UXy(mm) s_ux[me64 + (mm) * TX]
+#define WZy(mm) YS(s_wz, mm)
+#define UXz(mm) wux[4 + (mm)]
+#define VYz(mm) wvy[4 + (mm)]
+            // cpu/diffterm.cpp:159-187
+            const double uxx = D2(CUx) * dxi0s;
+            const double uyy = D2(CUy) * dxi1s;
+            const double uzz = D2(CUz) * dxi2s;
+            const double vyx = D1(VYx) * dxi0;
+            const double wzx = D1(WZx) * dxi0;
+            const double dmx = eta * (FourThirds * uxx + uyy + uzz + OneThird * (vyx + wzx));
+            // :198-226
+            const double vxx = D2(CVx) * dxi0s;
+            const double vyy = D2(CVy) * dxi1s;
+            const double vzz = D2(CVz) * dxi2s;
+            const double uxy = D1(UXy) * dxi1;
+            const double wzy = D1(WZy) * dxi1;
+            const double dmy = eta * (vxx + FourThirds * vyy + vzz + OneThird * (uxy + wzy));
+            // :237-265
+            const double wxx = D2(CWx) * dxi0s;
+            const double wyy = D2(CWy) * dxi1s;
+            const double wzz = D2(CWz) * dxi2s;
+            const double uxz = D1(UXz) * dxi2;
+            const double vyz = D1(VYz) * dxi2;
+            const double dmz = eta * (wxx + wyy + FourThirds * wzz + OneThird * (uxz + vyz));
+            // :276-311
+            const double txx = D2(CTx) * dxi0s;
+            const double tyy = D2(CTy) * dxi1s;
+            const double tzz = D2(CTz) * dxi2s;
+            const double ux = wux[4], vy = wvy[4], wz = s_wz[me];
+            const double uy = D1(CUy) * dxi1, uz = D1(CUz) * dxi2;
+            const double vx = D1(CVx) * dxi0, vz = D1(CVz) * dxi2;
+            const double wx = D1(CWx) * dxi0, wy = D1(CWy) * dxi1;
+            const double divu = TwoThirds * (ux + vy + wz);
+            const double tauxx = 2.E0 * ux - divu;
+            const double tauyy = 2.E0 * vy - divu;
+            const double tauzz = 2.E0 * wz - divu;
+            const double tauxy = uy + vx;
+            const double tauxz = uz + wx;
+            const double tauyz = vz + wy;
+            double mechwork = tauxx * ux + tauyy * vy + tauzz * wz + (tauxy * tauxy) + (tauxz * tauxz)
+                              + (tauyz * tauyz);
+            mechwork = eta * mechwork + dmx * wu[4] + dmy * wv[4] + dmz * ww[4];
+            const int64_t fo = iidx(g, gi - NG, gj - NG, k);
+            const int64_t c = g.c;
+            difflux[fo] = 0.0E0;
+            difflux[fo + IMX * c] = dmx;
+            difflux[fo + IMY * c] = dmy;
+            difflux[fo + IMZ * c] = dmz;
+            difflux[fo + IENE * c] = alam * (txx + tyy + tzz) + mechwork;
+#undef XS
+#undef YS
+#undef CUx
+#undef CUy
+#undef CUz
+#undef CVx
+#undef CVy
+#undef CVz
+#undef CWx
+#undef CWy
+#undef CWz
+#undef CTx
+#undef CTy
+#undef CTz
+#undef VYx
+#undef WZx
+#undef UXy
+#undef WZy
+#undef UXz
+#undef VYz
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            wu[m] = wu[m + 1];
+            wv[m] = wv[m + 1];
+            ww[m] = ww[m + 1];
+            wt[m] = wt[m + 1];
+            wux[m] = wux[m + 1];
+            wvy[m] = wvy[m + 1];
+        }
+    }
+}
+
+// z runs: enough workgroups to fill 256 CUs several times over (one 512-thread
+// workgroup per CU fits the LDS), with runs long enough that the 8-plane window
+// warm-up stays a small overhead.
+int pipe_kz(const Geom &g, int tiles_xy)
+{
+    int kz = g.n2;
+    while (kz > 16 && (int64_t)tiles_xy * ((g.n2 + kz - 1) / kz) < 2048) kz = (kz + 1) / 2;
+    return kz;
+}
+
+}  // namespace
+
+bool pipe_aligned(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *q, double *flux,
+                               double dxi0, double dxi1, double dxi2, const double *d,
+                               const double *base, double *out, const double *dt_dev, int phase,
+                               hipStream_t s)
+{
+    const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
+    const int kz = pipe_kz(g, ntx * nty);
+    const int nb = ntx * nty * ((g.n2 + kz - 1) / kz);
+    PipeRk rk{d, base, out, dt_dev, phase};
+    if (phase)
+        hypterm_pipe_kernel<true><<<nb, NT, 0, s>>>(g, cons, q, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk);
+    else
+        hypterm_pipe_kernel<false><<<nb, NT, 0, s>>>(g, cons, q, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk);
+    return hipGetLastError();
+}
+
+hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hipStream_t s)
+{
+    const int kz = g.n2 > 128 ? 128 : g.n2;
+    const dim3 grid((unsigned)((g.pg + 255) / 256), (unsigned)((g.n2 + kz - 1) / kz));
+    wz_kernel<<<grid, 256, 0, s>>>(g, q + QW * g.cg, wz, dxi2, kz);
+    return hipGetLastError();
+}
+
+hipError_t launch_diffterm_pipe(const Geom &g, const double *q, const double *wz, double *difflux,
+                                double dxi0, double dxi1, double dxi2, double eta, double alam,
+                                hipStream_t s)
+{
+    const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
+    const int kz = pipe_kz(g, ntx * nty);
+    const int nb = ntx * nty * ((g.n2 + kz - 1) / kz);
+    diffterm_pipe_kernel<<<nb, NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty);
+    return hipGetLastError();
+}
+
+}  // namespace cns

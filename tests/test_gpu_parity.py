@@ -16,7 +16,10 @@ from conftest import golden
 
 pytestmark = pytest.mark.gpu
 
-IMPLS = {"tiled": _lib.CNS_IMPL_TILED, "direct": _lib.CNS_IMPL_DIRECT}
+# "tiled" runs the DMA-pipelined kernels (the default for aligned arrays),
+# "staged" the register-staged tiled kernels (CNS_PIPE=0), "direct" the
+# one-thread-per-cell restatement.
+IMPLS = {"tiled": _lib.CNS_IMPL_TILED, "staged": _lib.CNS_IMPL_TILED, "direct": _lib.CNS_IMPL_DIRECT}
 SIZES = [(16, 16, 16), (20, 12, 17), (64, 64, 64), (70, 33, 40), (4, 4, 4), (130, 9, 5)]
 
 
@@ -37,8 +40,10 @@ def _normwise(a, b):
 
 
 @pytest.fixture(params=list(IMPLS))
-def impl(request, device):
+def impl(request, device, monkeypatch):
     ops.set_stencil_impl(IMPLS[request.param])
+    if request.param == "staged":
+        monkeypatch.setenv("CNS_PIPE", "0")
     yield request.param
     ops.set_stencil_impl(_lib.CNS_IMPL_TILED)
 
