@@ -157,6 +157,144 @@ def cpu_baseline(edge: int = 128, budget_s: float = 15.0):
                       f"{el:.1f} s; the path has no threading (its OpenMP is commented out)"}
 
 
+BATCH_EDGES = (16, 24, 32, 48, 64, 96)
+
+
+def batched_requests(nbox: int = 64, steps: int = 50, seed: int = 4):
+    """64 boxes, edges uniform over {16,...,96}, Mach U[0.05, 0.3], own seeded states."""
+    import numpy as np
+    from cns_nospec_amd.serve import Request
+    rng = np.random.default_rng(seed)
+    edges = rng.choice(BATCH_EDGES, nbox)
+    machs = rng.uniform(0.05, 0.3, nbox)
+    seeds = rng.integers(0, 2 ** 31 - 1, nbox)
+    return [Request(i, 0.0, int(edges[i]), steps, int(seeds[i]), float(machs[i])) for i in range(nbox)]
+
+
+def field_errors(a, b):
+    """Per-field relative L2 and normwise max error of a vs b, (5, ...) arrays."""
+    import numpy as np
+    out = {}
+    for l, name in enumerate(("rho", "mx", "my", "mz", "E")):
+        x, y = a[l].ravel(), b[l].ravel()
+        out[name] = {"rel_l2": float(np.linalg.norm(x - y) / np.linalg.norm(y)),
+                     "max_rel": float(np.abs(x - y).max() / np.abs(y).max())}
+    return out
+
+
+def worst_errors(errs):
+    return {f: {k: max(e[f][k] for e in errs) for k in ("rel_l2", "max_rel")} for f in errs[0]}
+
+
+def run_batched(reqs, fp32: bool):
+    """All boxes resident in one engine; returns rate and final states (host)."""
+    import torch
+    from cns_nospec_amd.batch import BatchEngine, box_bytes
+    from cns_nospec_amd.serve import request_state
+    budget = sum(box_bytes(r.edge, fp32) for r in reqs) + (256 << 20)
+    eng = BatchEngine(budget, fp32=fp32, max_boxes=len(reqs))
+    ids = [eng.add(request_state(r).reshape(-1), r.edge) for r in reqs]
+    eng.step(0)  # captures the step graph
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.step(reqs[0].steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    q = eng.query(ids)
+    states = [eng.read(i).cpu().numpy() for i in ids]
+    eng.close()
+    torch.cuda.empty_cache()
+    work = sum(r.edge ** 3 * r.steps for r in reqs)
+    return {"cell_steps_per_s": work / el, "elapsed_s": el,
+            "failed": sum(1 for i in ids if q[i]["err"])}, states
+
+
+def run_plain_boxes(reqs):
+    """The same boxes one after another through cns_advance (dt to the host each step)."""
+    import torch
+    from cns_nospec_amd import ops
+    from cns_nospec_amd.layout import NC, BoxLayout
+    from cns_nospec_amd.serve import request_state
+    states, el = [], 0.0
+    for r in reqs:
+        lay = BoxLayout.cube(r.edge)
+        U, Unew, Q, D, F = ops.allocate_variables(lay, "cuda")
+        g, ng, e = lay.dimg, lay.ng, r.edge
+        view = U.view(NC, g[2], g[1], lay.xg)[:, ng:ng + e, ng:ng + e, ng:ng + e]
+        view.copy_(request_state(r))
+        st = ops.Stepper(lay)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(r.steps):
+            st.new_advance(U, Unew, Q, D, F)
+        torch.cuda.synchronize()
+        el += time.perf_counter() - t0
+        states.append(view.cpu().numpy())
+        st.close()
+        del U, Unew, Q, D, F
+    work = sum(r.edge ** 3 * r.steps for r in reqs)
+    return {"cell_steps_per_s": work / el, "elapsed_s": el}, states
+
+
+def batched_secondary():
+    reqs = batched_requests()
+    plain, s64 = run_plain_boxes(reqs)
+    b64, sb64 = run_batched(reqs, False)
+    b32, sb32 = run_batched(reqs, True)
+    import numpy as np
+    exact = all(np.array_equal(a, b) for a, b in zip(sb64, s64))
+    errs = worst_errors([field_errors(a, b) for a, b in zip(sb32, s64)])
+    work = sum(r.edge ** 3 * r.steps for r in reqs)
+    return {"workload": "64 periodic boxes, edges uniform over {16,24,32,48,64,96} (seed 4), "
+                        "Mach U[0.05,0.3], own seeded sinusoidal states, 50 RK3 steps each",
+            "cell_steps": work, "unit": "cell-steps/s",
+            "plain_fp64": plain, "batched_fp64": b64, "batched_fp32_flux": b32,
+            "batched_fp64_bit_exact_vs_plain": exact,
+            "ratio_batched_fp32_vs_plain_fp64": b32["cell_steps_per_s"] / plain["cell_steps_per_s"],
+            "ratio_batched_fp64_vs_plain_fp64": b64["cell_steps_per_s"] / plain["cell_steps_per_s"],
+            "fp32_flux_error_vs_fp64_after_50_steps": errs}
+
+
+def served_secondary(nreq: int, seed: int = 2024):
+    """The mixed-size request stream (edge 24-128, 50-500 steps) served twice: as a
+    burst (every request waiting at t = 0: both servers saturated, so the aggregate
+    rates compare capacities), and as a Poisson stream offering 1.2x the plain path's
+    measured capacity (latency under load)."""
+    import numpy as np
+    import torch
+    from cns_nospec_amd.serve import BatchServer, PlainServer, make_stream
+    budget = 200 << 30
+    keep = tuple(range(min(4, nreq)))
+    burst = make_stream(nreq, 0.0, seed)
+    work = sum(r.edge ** 3 * r.steps for r in burst)
+    res_p, plain = PlainServer(keep_states=keep).run(burst)
+    torch.cuda.empty_cache()
+    srv = BatchServer(budget, fp32=True, keep_states=keep)
+    res_b, bat = srv.run(burst)
+    srv.close()
+    torch.cuda.empty_cache()
+    errs = worst_errors([field_errors(res_b[i].state.cpu().numpy(), res_p[i].state.cpu().numpy())
+                         for i in keep])
+    lam = 1.2 * plain["cell_steps_per_s"] / (work / nreq)
+    stream = make_stream(nreq, lam, seed)
+    _, plain_l = PlainServer().run(stream)
+    torch.cuda.empty_cache()
+    srv = BatchServer(budget, fp32=True)
+    _, bat_l = srv.run(stream)
+    srv.close()
+    torch.cuda.empty_cache()
+    return {"workload": f"{nreq} requests (seed {seed}), edge U{{24..128}}, steps U{{50..500}}, "
+                        "Mach U[0.05,0.3], 200 GB budget, FP32-flux continuous batching vs "
+                        "plain FP64 one box at a time",
+            "cell_steps": int(work), "unit": "cell-steps/s",
+            "burst": {"plain_fp64": plain, "batched_fp32_flux": bat,
+                      "ratio": bat["cell_steps_per_s"] / plain["cell_steps_per_s"]},
+            "poisson": {"rate_per_s": lam, "offered_vs_plain_capacity": 1.2,
+                        "plain_fp64": plain_l, "batched_fp32_flux": bat_l},
+            "fp32_flux_error_vs_fp64": errs,
+            "error_sample": f"requests {list(keep)} of the burst run, final state"}
+
+
 def kernels_once(edge: int, reps: int = 3):
     """One warm-up and `reps` launches of every kernel of the fused plain step on an
     edge^3 box, plus a plain device copy of known size (for calibrating rocprofv3
@@ -189,6 +327,8 @@ def main():
     ap.add_argument("--edge", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=200,
+                    help="requests of the served workload (0 skips it)")
     ap.add_argument("--kernels-once", action="store_true",
                     help="run each step kernel a few times and a calibration copy (for rocprofv3)")
     args = ap.parse_args()
@@ -251,6 +391,9 @@ def main():
         sec["plain_64"] = {"value": r64["cell_steps_per_s"], "unit": "cell-steps/s",
                            "ms_per_step": r64["ms_per_step"], "steps": 100,
                            "vs_c2050_published": r64["cell_steps_per_s"] / C2050_64_CELL_STEPS}
+        sec["batched"] = batched_secondary()
+        if args.serve_requests > 0:
+            sec["served"] = served_secondary(args.serve_requests)
         out["secondary"] = sec
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
