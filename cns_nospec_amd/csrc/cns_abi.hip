@@ -142,16 +142,19 @@ hipError_t stencil_hypterm(const Geom &g, const double *cons, const double *q, d
         return launch_hypterm_direct(g, cons, q, flux, dxi[0], dxi[1], dxi[2], s);
     if (pipe_on({cons, q, flux}))
         return launch_hypterm_pipe(g, cons, q, flux, dxi[0], dxi[1], dxi[2], nullptr, nullptr,
-                                   nullptr, nullptr, 0, s);
+                                   nullptr, nullptr, 0, true, s);
     return launch_hypterm_tiled(g, cons, q, flux, dxi[0], dxi[1], dxi[2], s);
 }
 
+// write_f: whether F must hold this stage's flux afterwards (only the last stage's
+// F is an output of the step; the register-staged kernel always stores it).
 hipError_t stencil_hypterm_rk(const Geom &g, const double *cons, const double *q, double *flux,
                               const double dxi[3], const double *d, const double *base, double *out,
-                              const double *dt, int phase, hipStream_t s)
+                              const double *dt, int phase, bool write_f, hipStream_t s)
 {
     if (pipe_on({cons, q, flux, d, base, out}))
-        return launch_hypterm_pipe(g, cons, q, flux, dxi[0], dxi[1], dxi[2], d, base, out, dt, phase, s);
+        return launch_hypterm_pipe(g, cons, q, flux, dxi[0], dxi[1], dxi[2], d, base, out, dt, phase,
+                                   write_f, s);
     return launch_hypterm_rk(g, cons, q, flux, dxi[0], dxi[1], dxi[2], d, base, out, dt, phase, s);
 }
 
@@ -449,17 +452,17 @@ int cns_advance_async(cns_ctx *ctx, double *U, double *Unew, double *Q, double *
         TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, U, Q, ctx->d_acc, ctx->dx[0], ctx->dx[1], ctx->dx[2], s));
         HIPCHK(launch_courno_to_dt(ctx->d_acc, ctx->cfl, dt, courno_dev ? courno_dev : ctx->d_courno, s));
         TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
-        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, stencil_hypterm_rk(g, U, Q, F, ctx->dxi, D, U, T, dt, 1, s));
+        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, stencil_hypterm_rk(g, U, Q, F, ctx->dxi, D, U, T, dt, 1, false, s));
 
         TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, T, s));
         TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, T, Q, nullptr, 0.0, 0.0, 0.0, s));
         TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
-        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, stencil_hypterm_rk(g, T, Q, F, ctx->dxi, D, U, Unew, dt, 2, s));
+        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, stencil_hypterm_rk(g, T, Q, F, ctx->dxi, D, U, Unew, dt, 2, false, s));
 
         TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, Unew, s));
         TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, Unew, Q, nullptr, 0.0, 0.0, 0.0, s));
         TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
-        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, stencil_hypterm_rk(g, Unew, Q, F, ctx->dxi, D, U, U, dt, 3, s));
+        TIMED(ctx, CNS_PROF_HYPTERM_RK, s, stencil_hypterm_rk(g, Unew, Q, F, ctx->dxi, D, U, U, dt, 3, true, s));
         return CNS_OK;
     }
     // Unfused composition, kernel for kernel as the reference.

@@ -45,13 +45,15 @@ hipError_t launch_diffterm_tiled(const Geom &g, const double *q, double *difflux
 
 // DMA-pipelined z-marching stencils (cns_stencil_pipe.hip).  Every array must be
 // 16-byte aligned (pipe_aligned).  hypterm: phase 0 is plain hypterm, 1..3 fuse the
-// RK stage update as launch_hypterm_rk.  diffterm reads wz, a one-component ghosted
-// array filled by launch_wz from the same q.
+// RK stage update as launch_hypterm_rk (phase 1 with base == cons takes the direct
+// form); write_f = false skips storing flux (for a stage whose F is overwritten
+// later).  diffterm reads wz, a one-component ghosted array filled by launch_wz from
+// the same q.
 bool pipe_aligned(const void *p);
 hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *q, double *flux,
                                double dxi0, double dxi1, double dxi2, const double *d,
                                const double *base, double *out, const double *dt_dev, int phase,
-                               hipStream_t s);
+                               bool write_f, hipStream_t s);
 hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hipStream_t s);
 hipError_t launch_diffterm_pipe(const Geom &g, const double *q, const double *wz, double *difflux,
                                 double dxi0, double dxi1, double dxi2, double eta, double alam,

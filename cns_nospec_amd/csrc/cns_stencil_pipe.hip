@@ -114,10 +114,16 @@ struct PipeRk {
 
 // ======================================================================= hypterm
 // LDS fields (centre plane): mx, my, mz, E, p, u, v.  Register windows: mx, my, mz,
-// E, w, p.  With RK the stage update of the cell is applied as in cns_stencil.hip.
+// E, w, p.  With RK the stage update of the cell is applied as in cns_stencil.hip,
+// with three savings of HBM traffic that leave every result bit-identical:
+//  * D_rho is not read: diffterm always stores 0.0 there (cpu/diffterm.cpp:63-68);
+//  * STAGE1: the base is cons itself, so the update is cons + dt*(D+F), the
+//    reference's own form (cpu/advance.cpp:82), without re-reading the base;
+//  * !WRITE_F: flux is not stored (stages 1 and 2 of a step, whose F the third
+//    stage overwrites).
 enum { HMX = 0, HMY, HMZ, HE, HP, HU, HV, HNF };
 
-template <bool RK>
+template <bool RK, bool STAGE1, bool WRITE_F>
 __global__ void __launch_bounds__(NT, 2)
 hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__restrict__ q,
                     double *__restrict__ flux, double dxi0, double dxi1, double dxi2, int kz,
@@ -182,10 +188,11 @@ hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__res
         double rkd[5], rkb[5], rho_c = 0.0;
         if (RK && active) {
             rho_c = cons[zo];
+            rkd[IRHO] = 0.0E0;
 #pragma unroll
             for (int l = 0; l < NC; l++) {
-                rkd[l] = rk.d[l * g.c + fo];
-                rkb[l] = rk.base[l * cg + zo];
+                if (l != IRHO) rkd[l] = rk.d[l * g.c + fo];
+                if (!STAGE1) rkb[l] = rk.base[l * cg + zo];
             }
         }
 
@@ -248,13 +255,16 @@ hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__res
 #undef ZE
 #undef ZW
 #undef ZP
+            if (WRITE_F) {
 #pragma unroll
-            for (int l = 0; l < NC; l++) flux[l * g.c + fo] = F[l];
+                for (int l = 0; l < NC; l++) flux[l * g.c + fo] = F[l];
+            }
             if (RK) {
                 const double uc[5] = {rho_c, wmx[4], wmy[4], wmz[4], wE[4]};
 #pragma unroll
                 for (int l = 0; l < NC; l++)
-                    rk.out[l * cg + zo] = ra * rkb[l] + rb * (uc[l] + dt * (rkd[l] + F[l]));
+                    rk.out[l * cg + zo] = STAGE1 ? uc[l] + dt * (rkd[l] + F[l])
+                                                 : ra * rkb[l] + rb * (uc[l] + dt * (rkd[l] + F[l]));
             }
         }
 #pragma unroll
@@ -525,16 +535,22 @@ bool pipe_aligned(const void *p) { return ((uintptr_t)p & 15) == 0; }
 hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *q, double *flux,
                                double dxi0, double dxi1, double dxi2, const double *d,
                                const double *base, double *out, const double *dt_dev, int phase,
-                               hipStream_t s)
+                               bool write_f, hipStream_t s)
 {
     const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
     const int kz = pipe_kz(g, ntx * nty);
     const int nb = ntx * nty * ((g.n2 + kz - 1) / kz);
     PipeRk rk{d, base, out, dt_dev, phase};
-    if (phase)
-        hypterm_pipe_kernel<true><<<nb, NT, 0, s>>>(g, cons, q, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk);
+    const bool stage1 = phase == 1 && base == cons;
+#define HYP_PIPE(RK, S1, WF) \
+    hypterm_pipe_kernel<RK, S1, WF><<<nb, NT, 0, s>>>(g, cons, q, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk)
+    if (!phase)
+        HYP_PIPE(false, false, true);
+    else if (stage1)
+        write_f ? HYP_PIPE(true, true, true) : HYP_PIPE(true, true, false);
     else
-        hypterm_pipe_kernel<false><<<nb, NT, 0, s>>>(g, cons, q, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk);
+        write_f ? HYP_PIPE(true, false, true) : HYP_PIPE(true, false, false);
+#undef HYP_PIPE
     return hipGetLastError();
 }
 
