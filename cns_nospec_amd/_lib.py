@@ -95,6 +95,7 @@ _SIGS = {
     "cns_batch_set_tile": (_INT, [_VP, _INT, _INT, _INT]),
     "cns_batch_get_tile": (_INT, [_VP, _INT, C.POINTER(_INT), C.POINTER(_INT)]),
     "cns_batch_stats": (_INT, [_VP, C.POINTER(C.c_size_t), C.POINTER(_INT), C.POINTER(_INT)]),
+    "cns_batch_set_target": (_INT, [_VP, _INT, C.c_longlong]),
     "cns_read_general_input": (_INT, [C.c_char_p, C.POINTER(Config)]),
     "cns_write_general_input": (_INT, [C.c_char_p, C.POINTER(Config)]),
     "cns_read_field": (_INT, [C.c_char_p, _I3, _INT, _INT, _VP]),

@@ -90,6 +90,11 @@ class BatchEngine:
         _lib.check(lib.cns_batch_read(self._h, box_id, C.c_void_p(out.data_ptr())))
         return out
 
+    def set_target(self, box_id: int, steps: int):
+        """Stop advancing the box once it has taken `steps` steps (0: no limit)."""
+        lib = _lib_stream()
+        _lib.check(lib.cns_batch_set_target(self._h, box_id, int(steps)))
+
     def step(self, nsteps: int = 1):
         lib = _lib_stream()
         _lib.check(lib.cns_batch_step(self._h, int(nsteps)))

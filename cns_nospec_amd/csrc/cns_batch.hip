@@ -145,6 +145,7 @@ __global__ void batch_finalize_kernel(BoxDev *__restrict__ boxes, const int *__r
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     BoxDev &B = boxes[ids[t]];
+    if (box_stopped(B)) return;
     const double c = __longlong_as_double((long long)B.acc);
     double courno = c > 1.0E-50 ? c : 1.0E-50;
     if (!(c == c)) courno = c;
@@ -608,6 +609,7 @@ batch_diff_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles,
     __shared__ FT lds[9 * kLdsCells];
     for (int ti = xcd_rank(blockIdx.x, gridDim.x); ti < ntiles; ti += gridDim.x) {
         const TileArgs a = read_tile(tiles, ti);
+        if (box_stopped(boxes[a.b])) continue;  // uniform over the workgroup
         diff_tile<FT, LTX>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, stage, lds);
     }
 }
@@ -619,6 +621,7 @@ batch_hyp_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, 
     __shared__ FT lds[7 * kLdsCells];
     for (int ti = xcd_rank(blockIdx.x, gridDim.x); ti < ntiles; ti += gridDim.x) {
         const TileArgs a = read_tile(tiles, ti);
+        if (box_stopped(boxes[a.b])) continue;  // uniform over the workgroup
         const unsigned long long cm = hyp_tile<FT, LTX>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, stage, lds);
         if (stage == 3) {
             const unsigned long long m = block_max_u64(cm);

@@ -25,7 +25,11 @@ struct BoxDev {
     double dt;      // dt of the current step
     double courno;  // its Courant number
     long long steps;
+    long long target;  // the box stops advancing once steps reaches it (0: no limit)
 };
+
+// A box that has reached its step target is skipped by every kernel of the step.
+__host__ __device__ inline bool box_stopped(const BoxDev &b) { return b.target > 0 && b.steps >= b.target; }
 
 struct TileDev {
     int box;

@@ -404,6 +404,19 @@ int cns_batch_step(cns_batch *b, int nsteps)
     return CNS_OK;
 }
 
+int cns_batch_set_target(cns_batch *b, int box_id, long long target)
+{
+    int rc = check_id(b, box_id);
+    if (rc) return rc;
+    if (target < 0) return fail(CNS_EINVAL, "negative step target");
+    hipStream_t s = current_stream();
+    b->hrec[box_id].target = target;
+    CNS_HIPCHK(hipMemcpyAsync(&b->d_boxes[box_id].target, &b->hrec[box_id].target, sizeof(long long),
+                              hipMemcpyHostToDevice, s));
+    CNS_HIPCHK(hipStreamSynchronize(s));  // the staging record may change before the copy runs
+    return CNS_OK;
+}
+
 int cns_batch_query(cns_batch *b, int n, const int *ids, long long *steps, double *dt, double *courno,
                     int *err)
 {

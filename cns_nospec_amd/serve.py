@@ -142,13 +142,15 @@ class BatchServer:
             now = clock.now()
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            # admission: FIFO while the budget has room
+            # admission: FIFO while the budget has room; each box stops by itself on the
+            # device once it has taken its requested steps
             while waiting:
                 req = waiting[0]
                 try:
                     bid = eng.add(request_state(req).reshape(-1), req.edge)
                 except _lib.CnsBudgetError:
                     break
+                eng.set_target(bid, req.steps)
                 waiting.popleft()
                 res = Result(req.rid, req.edge, req.steps, req.arrival, admitted=clock.now())
                 results[req.rid] = res
@@ -160,14 +162,16 @@ class BatchServer:
                 if waiting:
                     raise RuntimeError("a single request does not fit in the engine's budget")
                 break
-            k = min(self.max_chunk, min(v[1] for v in active.values()))
+            # a chunk of steps: boxes that reach their target mid-chunk stop there, so the
+            # chunk need not end at the first box's end (that would cost a sync and a
+            # work-list rebuild per finishing box)
+            k = min(self.max_chunk, max(v[1] for v in active.values()))
             eng.step(k)
-            torch.cuda.synchronize()
-            for bid in list(active):
-                active[bid][1] -= k
-            finished = [bid for bid, v in active.items() if v[1] == 0]
+            q = eng.query(list(active))  # synchronises
+            for bid, v in active.items():
+                v[1] = v[0].steps - q[bid]["steps"]
+            finished = [bid for bid, v in active.items() if v[1] <= 0 or q[bid]["err"]]
             if finished:
-                q = eng.query(finished)
                 t = clock.now()
                 for bid in finished:
                     req, _, res = active.pop(bid)

@@ -60,6 +60,13 @@ int cns_batch_read(cns_batch *b, int box_id, double *u);
 /* Advance every resident box by nsteps RK3 steps (graph replays, asynchronous). */
 int cns_batch_step(cns_batch *b, int nsteps);
 
+/* Step target of a box: once its step count (steps since admission) reaches
+ * target, every later cns_batch_step leaves the box untouched (its state, dt
+ * and count stay as they are), so a server can replay many steps without
+ * stopping at each box's end.  target = 0 removes the limit.  No counterpart in
+ * the reference. */
+int cns_batch_set_target(cns_batch *b, int box_id, long long target);
+
 /* Per-box status for n ids (synchronises the stream): steps taken since
  * admission, dt of the last step, its Courant number, and err = 1 when a
  * non-finite Courant number was seen (the box's state is not physical; the

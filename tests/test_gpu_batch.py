@@ -66,6 +66,35 @@ def test_alone_equals_in_mix(device, fp32):
     mix.close()
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+def test_step_target_stops_a_box_exactly(device, fp32):
+    """A box with a step target stops there: its state equals the same box advanced
+    exactly that many steps, while the other boxes keep going; a target of 0 lifts
+    the limit again."""
+    u = compact_sinusoid(24, 8, mach=0.2)
+    v = compact_sinusoid(32, 9, mach=0.1)
+    ref = BatchEngine(1 << 30, fp32=fp32, max_boxes=2)
+    r = ref.add(u.reshape(-1), 24)
+    ref.step(3)
+    u3 = ref.read(r).cpu().numpy()
+    ref.step(2)
+    u5 = ref.read(r).cpu().numpy()
+    eng = BatchEngine(1 << 30, fp32=fp32, max_boxes=4)
+    a = eng.add(u.reshape(-1), 24)
+    b = eng.add(v.reshape(-1), 32)
+    eng.set_target(a, 3)
+    eng.step(7)
+    q = eng.query([a, b])
+    assert q[a]["steps"] == 3 and q[b]["steps"] == 7
+    assert np.array_equal(eng.read(a).cpu().numpy(), u3)
+    eng.set_target(a, 0)
+    eng.step(2)
+    assert eng.query([a])[a]["steps"] == 5
+    assert np.array_equal(eng.read(a).cpu().numpy(), u5)
+    ref.close()
+    eng.close()
+
+
 def test_fp32_flux_error_after_100_steps(device):
     edge, nsteps = 32, 100
     u = compact_sinusoid(edge, 5, mach=0.085)
