@@ -50,9 +50,10 @@ def algorithmic_bytes(kind: str, n: int, stage: int = 2) -> float:
         "diffterm": 4 * g3 + 5 * n3,
         "hypterm": 8 * g3 + 5 * n3,
         "rk": rk,
-        # the fused pass's own minimum: hypterm's reads and F, plus D, the stage base and
-        # rho of the cell in, the stage result out (F is not re-read)
-        "hypterm_rk": 8 * g3 + 21 * n3,
+        # the fused pass, averaged over a step's three stages: cons and q over their halos,
+        # the cell's rho and D without its zero rho row (5c), the stage base in stages 2-3
+        # (10c / 3), the result (5c) and F in stage 3 only (5c / 3)
+        "hypterm_rk": 8 * g3 + 15 * n3,
         "fill_ctoprim": 10 * (g3 - n3) + 11 * g3,
     }
     return 8.0 * table[kind]

@@ -12,8 +12,9 @@ hipError_t launch_fill_boundary(const Geom &g, double *u, hipStream_t s);
 hipError_t launch_courno_reset(unsigned long long *acc, double init, hipStream_t s);
 
 // ctoprim over the ghosted box; acc != nullptr adds the Courant max reduction.
+// full = false stores only u, v, w and T (what diffterm reads), not rho and p.
 hipError_t launch_ctoprim(const Geom &g, const double *u, double *q, unsigned long long *acc,
-                          double dx0, double dx1, double dx2, hipStream_t s);
+                          double dx0, double dx1, double dx2, hipStream_t s, bool full = true);
 
 // dt = cfl / max(acc, 1e-50) into dt_dev (and the Courant number into courno_dev).
 hipError_t launch_courno_to_dt(const unsigned long long *acc, double cfl, double *dt_dev,
@@ -54,6 +55,11 @@ hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *
                                double dxi0, double dxi1, double dxi2, const double *d,
                                const double *base, double *out, const double *dt_dev, int phase,
                                bool write_f, hipStream_t s);
+// The same fused stage reading only cons (phases 1..3): u, v, w, p are derived from it
+// with ctoprim's arithmetic, so it equals launch_hypterm_pipe with q = ctoprim(cons).
+hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, double dxi0,
+                               double dxi1, double dxi2, const double *d, const double *base,
+                               double *out, const double *dt_dev, int phase, bool write_f, hipStream_t s);
 hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hipStream_t s);
 hipError_t launch_diffterm_pipe(const Geom &g, const double *q, const double *wz, double *difflux,
                                 double dxi0, double dxi1, double dxi2, double eta, double alam,

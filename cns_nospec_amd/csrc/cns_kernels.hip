@@ -102,7 +102,7 @@ __global__ void courno_to_dt_kernel(const unsigned long long *acc, double cfl, d
 template <bool COURNO>
 __global__ void __launch_bounds__(kStreamBlock)
 ctoprim_kernel(Geom g, const double *__restrict__ u, double *__restrict__ q,
-               unsigned long long *__restrict__ acc, double dx0, double dx1, double dx2)
+               unsigned long long *__restrict__ acc, double dx0, double dx1, double dx2, bool full)
 {
     const int64_t total = (int64_t)g.g2 * g.pg;
     const int64_t cg = g.cg;
@@ -119,11 +119,11 @@ ctoprim_kernel(Geom g, const double *__restrict__ u, double *__restrict__ q,
         const double qw = u[e + 3 * cg] * rhoinv;
         const double eint = u[e + 4 * cg] * rhoinv - 0.5E0 * ((qu * qu) + (qv * qv) + (qw * qw));
         const double p = (GAMMA - 1.0E0) * eint * rho;
-        q[e] = rho;
+        if (full) q[e] = rho;
         q[e + cg] = qu;
         q[e + 2 * cg] = qv;
         q[e + 3 * cg] = qw;
-        q[e + 4 * cg] = p;
+        if (full) q[e + 4 * cg] = p;
         q[e + 5 * cg] = eint / CV;
         if (COURNO) {
             const int k = (int)(row / g.g1);
@@ -357,14 +357,14 @@ hipError_t launch_courno_reset(unsigned long long *acc, double init, hipStream_t
 }
 
 hipError_t launch_ctoprim(const Geom &g, const double *u, double *q, unsigned long long *acc,
-                          double dx0, double dx1, double dx2, hipStream_t s)
+                          double dx0, double dx1, double dx2, hipStream_t s, bool full)
 {
     const int64_t total = (int64_t)g.g2 * g.pg;
     const int grid = stream_grid(total, 8);
     if (acc)
-        ctoprim_kernel<true><<<grid, kStreamBlock, 0, s>>>(g, u, q, acc, dx0, dx1, dx2);
+        ctoprim_kernel<true><<<grid, kStreamBlock, 0, s>>>(g, u, q, acc, dx0, dx1, dx2, full);
     else
-        ctoprim_kernel<false><<<grid, kStreamBlock, 0, s>>>(g, u, q, nullptr, dx0, dx1, dx2);
+        ctoprim_kernel<false><<<grid, kStreamBlock, 0, s>>>(g, u, q, nullptr, dx0, dx1, dx2, full);
     return hipGetLastError();
 }
 

@@ -112,6 +112,105 @@ struct PipeRk {
     int phase;           // 0: plain hypterm
 };
 
+// hypterm's flux of the lane's cell (LDS index me of 16 x 72 tiles): x terms from the
+// row, y terms from the column, z terms from the register windows (plane k+m at
+// [4+m]), in the reference's order (cpu/hypterm.cpp:33-90, :101-147, :158-204).
+__device__ __forceinline__ void hyp_flux(const double *s_mx, const double *s_my, const double *s_mz,
+                                         const double *s_E, const double *s_p, const double *s_u,
+                                         const double *s_v, int me, const double (&wmx)[9],
+                                         const double (&wmy)[9], const double (&wmz)[9],
+                                         const double (&wE)[9], const double (&ww)[9],
+                                         const double (&wp)[9], double dxi0, double dxi1,
+                                         double dxi2, double (&F)[5])
+{
+    // ---- x: F = -(sum)*dxinv(1)
+#define XMX(m) s_mx[me + (m)]
+#define XMY(m) s_my[me + (m)]
+#define XMZ(m) s_mz[me + (m)]
+#define XE(m) s_E[me + (m)]
+#define XP(m) s_p[me + (m)]
+#define XU(m) s_u[me + (m)]
+    F[IRHO] = -(D1(XMX)) * dxi0;
+    F[IMX] = -(HMOMP(XMX, XU, XP)) * dxi0;
+    F[IMY] = -(HMOM(XMY, XU)) * dxi0;
+    F[IMZ] = -(HMOM(XMZ, XU)) * dxi0;
+    F[IENE] = -(HENE(XE, XU, XP)) * dxi0;
+    // ---- y: F = F - (sum)*dxinv(2)
+#define YMX(m) s_mx[me + (m) * PX]
+#define YMY(m) s_my[me + (m) * PX]
+#define YMZ(m) s_mz[me + (m) * PX]
+#define YE(m) s_E[me + (m) * PX]
+#define YP(m) s_p[me + (m) * PX]
+#define YV(m) s_v[me + (m) * PX]
+    F[IRHO] = F[IRHO] - (D1(YMY)) * dxi1;
+    F[IMX] = F[IMX] - (HMOM(YMX, YV)) * dxi1;
+    F[IMY] = F[IMY] - (HMOMP(YMY, YV, YP)) * dxi1;
+    F[IMZ] = F[IMZ] - (HMOM(YMZ, YV)) * dxi1;
+    F[IENE] = F[IENE] - (HENE(YE, YV, YP)) * dxi1;
+    // ---- z: from the register windows
+#define ZMX(m) wmx[4 + (m)]
+#define ZMY(m) wmy[4 + (m)]
+#define ZMZ(m) wmz[4 + (m)]
+#define ZE(m) wE[4 + (m)]
+#define ZW(m) ww[4 + (m)]
+#define ZP(m) wp[4 + (m)]
+    F[IRHO] = F[IRHO] - (D1(ZMZ)) * dxi2;
+    F[IMX] = F[IMX] - (HMOM(ZMX, ZW)) * dxi2;
+    F[IMY] = F[IMY] - (HMOM(ZMY, ZW)) * dxi2;
+    F[IMZ] = F[IMZ] - (HMOMP(ZMZ, ZW, ZP)) * dxi2;
+    F[IENE] = F[IENE] - (HENE(ZE, ZW, ZP)) * dxi2;
+#undef XMX
+#undef XMY
+#undef XMZ
+#undef XE
+#undef XP
+#undef XU
+#undef YMX
+#undef YMY
+#undef YMZ
+#undef YE
+#undef YP
+#undef YV
+#undef ZMX
+#undef ZMY
+#undef ZMZ
+#undef ZE
+#undef ZW
+#undef ZP
+}
+
+// u, v, w and p of one cell from its conserved values: ctoprim's arithmetic
+// (cpu/ctoprim.cpp:80-100; the same expressions as ctoprim_kernel), so the values are
+// bit-identical to what ctoprim stores in q.
+__device__ __forceinline__ void prim_uvwp(double rho, double mx, double my, double mz, double E,
+                                          double &u, double &v, double &w, double &p)
+{
+    const double rhoinv = 1.0E0 / rho;
+    u = mx * rhoinv;
+    v = my * rhoinv;
+    w = mz * rhoinv;
+    const double eint = E * rhoinv - 0.5E0 * ((u * u) + (v * v) + (w * w));
+    p = (GAMMA - 1.0E0) * eint * rho;
+}
+
+// The stage update of the cell and its stores (STAGE1: cons + dt(D+F), the reference's
+// form of cpu/advance.cpp:82; later stages ra*base + rb*(cons + dt(D+F)), which is
+// 3/4 base + 1/4(..) and 1/3 base + 2/3(..) of :114-116, :148-150).
+template <bool STAGE1, bool WRITE_F>
+__device__ __forceinline__ void rk_store(const double (&F)[5], const double (&uc)[5], const double (&rkd)[5],
+                                         const double (&rkb)[5], double ra, double rb, double dt,
+                                         double *flux, int64_t fo, int64_t c, double *out, int64_t zo,
+                                         int64_t cg)
+{
+    if (WRITE_F) {
+#pragma unroll
+        for (int l = 0; l < NC; l++) flux[l * c + fo] = F[l];
+    }
+#pragma unroll
+    for (int l = 0; l < NC; l++)
+        out[l * cg + zo] = STAGE1 ? uc[l] + dt * (rkd[l] + F[l]) : ra * rkb[l] + rb * (uc[l] + dt * (rkd[l] + F[l]));
+}
+
 // ======================================================================= hypterm
 // LDS fields (centre plane): mx, my, mz, E, p, u, v.  Register windows: mx, my, mz,
 // E, w, p.  With RK the stage update of the cell is applied as in cns_stencil.hip,
@@ -201,71 +300,131 @@ hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__res
                          *s_E = b + HE * TILE, *s_p = b + HP * TILE, *s_u = b + HU * TILE,
                          *s_v = b + HV * TILE;
             double F[5];
-            // ---- x (cpu/hypterm.cpp:33-90): F = -(sum)*dxinv(1)
-#define XMX(m) s_mx[me + (m)]
-#define XMY(m) s_my[me + (m)]
-#define XMZ(m) s_mz[me + (m)]
-#define XE(m) s_E[me + (m)]
-#define XP(m) s_p[me + (m)]
-#define XU(m) s_u[me + (m)]
-            F[IRHO] = -(D1(XMX)) * dxi0;
-            F[IMX] = -(HMOMP(XMX, XU, XP)) * dxi0;
-            F[IMY] = -(HMOM(XMY, XU)) * dxi0;
-            F[IMZ] = -(HMOM(XMZ, XU)) * dxi0;
-            F[IENE] = -(HENE(XE, XU, XP)) * dxi0;
-            // ---- y (cpu/hypterm.cpp:101-147): F = F - (sum)*dxinv(2)
-#define YMX(m) s_mx[me + (m) * PX]
-#define YMY(m) s_my[me + (m) * PX]
-#define YMZ(m) s_mz[me + (m) * PX]
-#define YE(m) s_E[me + (m) * PX]
-#define YP(m) s_p[me + (m) * PX]
-#define YV(m) s_v[me + (m) * PX]
-            F[IRHO] = F[IRHO] - (D1(YMY)) * dxi1;
-            F[IMX] = F[IMX] - (HMOM(YMX, YV)) * dxi1;
-            F[IMY] = F[IMY] - (HMOMP(YMY, YV, YP)) * dxi1;
-            F[IMZ] = F[IMZ] - (HMOM(YMZ, YV)) * dxi1;
-            F[IENE] = F[IENE] - (HENE(YE, YV, YP)) * dxi1;
-            // ---- z (cpu/hypterm.cpp:158-204) from the register windows
-#define ZMX(m) wmx[4 + (m)]
-#define ZMY(m) wmy[4 + (m)]
-#define ZMZ(m) wmz[4 + (m)]
-#define ZE(m) wE[4 + (m)]
-#define ZW(m) ww[4 + (m)]
-#define ZP(m) wp[4 + (m)]
-            F[IRHO] = F[IRHO] - (D1(ZMZ)) * dxi2;
-            F[IMX] = F[IMX] - (HMOM(ZMX, ZW)) * dxi2;
-            F[IMY] = F[IMY] - (HMOM(ZMY, ZW)) * dxi2;
-            F[IMZ] = F[IMZ] - (HMOMP(ZMZ, ZW, ZP)) * dxi2;
-            F[IENE] = F[IENE] - (HENE(ZE, ZW, ZP)) * dxi2;
-#undef XMX
-#undef XMY
-#undef XMZ
-#undef XE
-#undef XP
-#undef XU
-#undef YMX
-#undef YMY
-#undef YMZ
-#undef YE
-#undef YP
-#undef YV
-#undef ZMX
-#undef ZMY
-#undef ZMZ
-#undef ZE
-#undef ZW
-#undef ZP
-            if (WRITE_F) {
+            hyp_flux(s_mx, s_my, s_mz, s_E, s_p, s_u, s_v, me, wmx, wmy, wmz, wE, ww, wp, dxi0, dxi1, dxi2, F);
+            if (RK) {
+                const double uc[5] = {rho_c, wmx[4], wmy[4], wmz[4], wE[4]};
+                rk_store<STAGE1, WRITE_F>(F, uc, rkd, rkb, ra, rb, dt, flux, fo, g.c, rk.out, zo, cg);
+            } else {
 #pragma unroll
                 for (int l = 0; l < NC; l++) flux[l * g.c + fo] = F[l];
             }
-            if (RK) {
-                const double uc[5] = {rho_c, wmx[4], wmy[4], wmz[4], wE[4]};
+        }
 #pragma unroll
-                for (int l = 0; l < NC; l++)
-                    rk.out[l * cg + zo] = STAGE1 ? uc[l] + dt * (rkd[l] + F[l])
-                                                 : ra * rkb[l] + rb * (uc[l] + dt * (rkd[l] + F[l]));
+        for (int m = 0; m < 8; m++) {
+            wmx[m] = wmx[m + 1]; wmy[m] = wmy[m + 1]; wmz[m] = wmz[m + 1];
+            wE[m] = wE[m + 1]; ww[m] = ww[m + 1]; wp[m] = wp[m + 1];
+        }
+    }
+}
+
+// ====================================================== hypterm from the conserved state
+// The fused stage inside cns_advance, where q is ctoprim(cons) by construction: only
+// rho, mx, my, mz, E are staged by DMA, and u, v, p of the tile's cross cells are
+// derived into LDS with ctoprim's arithmetic after the plane barrier (a second
+// barrier publishes them); the windows' w and p are derived from the leading cell's
+// conserved values.  Three ghosted fields fewer to fetch per plane than the q-based
+// kernel, for the same bits.
+enum { CRHO = 0, CMX, CMY, CMZ, CE, CNF };
+enum { SDU = 0, SDV, SDP, SDNF };
+constexpr int XSTRIP = BY * PX;             // rows 4..11, all 72 columns
+constexpr int CROSS = XSTRIP + 8 * TX;      // + rows 0..3 and 12..15, columns 4..67
+
+template <bool STAGE1, bool WRITE_F>
+__global__ void __launch_bounds__(NT, 2)
+hypterm_cons_kernel(Geom g, const double *__restrict__ cons, double *__restrict__ flux, double dxi0,
+                    double dxi1, double dxi2, int kz, int ntx, int nty, PipeRk rk)
+{
+    __shared__ double sm[2 * CNF * TILE];
+    __shared__ double sd[SDNF * TILE];
+
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const TileDma td{bx * TX, by * BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
+    const int gi = td.gi0 + lane + 4, gj = td.gj0 + wave + 4;
+    const bool inbox = gi < g.g0 && gj < g.g1;
+    const bool active = gi < g.n0 + NG && gj < g.n1 + NG;
+    const int kbeg = bz * kz, kend = min(kbeg + kz, g.n2);
+    const int64_t cg = g.cg, pg = g.pg;
+    const double *c_rho = cons, *c_mx = cons + IMX * cg, *c_my = cons + IMY * cg,
+                 *c_mz = cons + IMZ * cg, *c_E = cons + IENE * cg;
+    const double *const src[CNF] = {c_rho, c_mx, c_my, c_mz, c_E};
+    const int64_t own = (int64_t)gj * g.xg + gi;
+    const int me = (wave + 4) * PX + lane + 4;
+    const double dt = rk.dt[0];
+    const double ra = rk.phase == 1 ? 0.0E0 : (rk.phase == 2 ? ThreeQuarters : OneThird);
+    const double rb = rk.phase == 1 ? 1.0E0 : (rk.phase == 2 ? OneQuarter : TwoThirds);
+
+    dma_plane<CNF>(src, (int64_t)(kbeg + NG) * pg, sm, td, wave, lane);
+
+    double wmx[9], wmy[9], wmz[9], wE[9], ww[9], wp[9];
+    double nrho = 1.0, nmx = 0, nmy = 0, nmz = 0, nE = 0;
+    if (inbox) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int64_t o = (int64_t)(kbeg + m) * pg + own;
+            wmx[m] = c_mx[o]; wmy[m] = c_my[o]; wmz[m] = c_mz[o]; wE[m] = c_E[o];
+            double u_, v_;
+            prim_uvwp(c_rho[o], wmx[m], wmy[m], wmz[m], wE[m], u_, v_, ww[m], wp[m]);
+        }
+        const int64_t o = (int64_t)(kbeg + 2 * NG) * pg + own;
+        nrho = c_rho[o]; nmx = c_mx[o]; nmy = c_my[o]; nmz = c_mz[o]; nE = c_E[o];
+    }
+
+    for (int k = kbeg; k < kend; k++) {
+        const int cur = (k - kbeg) & 1;
+        const double *b = sm + cur * (CNF * TILE);
+        plane_barrier();
+        if (k + 1 < kend) dma_plane<CNF>(src, (int64_t)(k + 1 + NG) * pg, sm + (cur ^ 1) * (CNF * TILE), td, wave, lane);
+
+        const int64_t zo = (int64_t)(k + NG) * pg + own;
+        {
+            wmx[8] = nmx; wmy[8] = nmy; wmz[8] = nmz; wE[8] = nE;
+            double u_, v_;
+            prim_uvwp(nrho, nmx, nmy, nmz, nE, u_, v_, ww[8], wp[8]);
+        }
+        if (inbox && k + 1 < kend) {
+            const int64_t o = zo + 5 * pg;
+            nrho = c_rho[o]; nmx = c_mx[o]; nmy = c_my[o]; nmz = c_mz[o]; nE = c_E[o];
+        }
+        const int64_t fo = iidx(g, gi - NG, gj - NG, k);
+        double rkd[5], rkb[5];
+        if (active) {
+            rkd[IRHO] = 0.0E0;
+#pragma unroll
+            for (int l = 0; l < NC; l++) {
+                if (l != IRHO) rkd[l] = rk.d[l * g.c + fo];
+                if (!STAGE1) rkb[l] = rk.base[l * cg + zo];
             }
+        }
+
+        // u, v, p of the cross cells of this plane's tile
+        for (int h = threadIdx.x; h < CROSS; h += NT) {
+            int r, c;
+            if (h < XSTRIP) {
+                r = 4 + h / PX;
+                c = h - (r - 4) * PX;
+            } else {
+                const int h2 = h - XSTRIP, rr = h2 >> 6;
+                r = rr < 4 ? rr : rr + BY;
+                c = 4 + (h2 & 63);
+            }
+            const int l = r * PX + c;
+            double u_, v_, w_, p_;
+            prim_uvwp(b[CRHO * TILE + l], b[CMX * TILE + l], b[CMY * TILE + l], b[CMZ * TILE + l],
+                      b[CE * TILE + l], u_, v_, w_, p_);
+            sd[SDU * TILE + l] = u_;
+            sd[SDV * TILE + l] = v_;
+            sd[SDP * TILE + l] = p_;
+        }
+        __syncthreads();
+
+        if (active) {
+            double F[5];
+            hyp_flux(b + CMX * TILE, b + CMY * TILE, b + CMZ * TILE, b + CE * TILE, sd + SDP * TILE,
+                     sd + SDU * TILE, sd + SDV * TILE, me, wmx, wmy, wmz, wE, ww, wp, dxi0, dxi1, dxi2, F);
+            const double uc[5] = {b[CRHO * TILE + me], wmx[4], wmy[4], wmz[4], wE[4]};
+            rk_store<STAGE1, WRITE_F>(F, uc, rkd, rkb, ra, rb, dt, flux, fo, g.c, rk.out, zo, cg);
         }
 #pragma unroll
         for (int m = 0; m < 8; m++) {
@@ -551,6 +710,25 @@ hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *
     else
         write_f ? HYP_PIPE(true, false, true) : HYP_PIPE(true, false, false);
 #undef HYP_PIPE
+    return hipGetLastError();
+}
+
+hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, double dxi0,
+                               double dxi1, double dxi2, const double *d, const double *base,
+                               double *out, const double *dt_dev, int phase, bool write_f, hipStream_t s)
+{
+    const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
+    const int kz = pipe_kz(g, ntx * nty);
+    const int nb = ntx * nty * ((g.n2 + kz - 1) / kz);
+    PipeRk rk{d, base, out, dt_dev, phase};
+    const bool stage1 = phase == 1 && base == cons;
+#define HYP_CONS(S1, WF) \
+    hypterm_cons_kernel<S1, WF><<<nb, NT, 0, s>>>(g, cons, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk)
+    if (stage1)
+        write_f ? HYP_CONS(true, true) : HYP_CONS(true, false);
+    else
+        write_f ? HYP_CONS(false, true) : HYP_CONS(false, false);
+#undef HYP_CONS
     return hipGetLastError();
 }
 
