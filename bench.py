@@ -256,6 +256,20 @@ def batched_secondary():
             "fp32_flux_error_vs_fp64_after_50_steps": errs}
 
 
+def fp32_error_secondary(nbox: int = 8, steps: int = 100, seed: int = 2024):
+    """FP32-flux error after exactly 100 steps on the first boxes of the request stream
+    (edge 24-128, Mach up to 0.3): FP32-flux engine against the FP64 engine, which is
+    bit-identical to the plain path and the oracle."""
+    from cns_nospec_amd.serve import make_stream
+    reqs = [r for r in make_stream(nbox, 0.0, seed)]
+    for r in reqs:
+        r.steps = steps
+    _, s64 = run_batched(reqs, False)
+    _, s32 = run_batched(reqs, True)
+    return {"boxes": [r.edge for r in reqs], "machs": [round(r.mach, 3) for r in reqs], "steps": steps,
+            "per_field_worst": worst_errors([field_errors(a, b) for a, b in zip(s32, s64)])}
+
+
 def served_secondary(nreq: int, seed: int = 2024):
     """The mixed-size request stream (edge 24-128, 50-500 steps) served twice: as a
     burst (every request waiting at t = 0: both servers saturated, so the aggregate
@@ -393,6 +407,7 @@ def main():
                            "ms_per_step": r64["ms_per_step"], "steps": 100,
                            "vs_c2050_published": r64["cell_steps_per_s"] / C2050_64_CELL_STEPS}
         sec["batched"] = batched_secondary()
+        sec["fp32_flux_error_100_steps"] = fp32_error_secondary()
         if args.serve_requests > 0:
             sec["served"] = served_secondary(args.serve_requests)
         out["secondary"] = sec
