@@ -446,26 +446,39 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
     const double pref = B.pref;
     const FT dxi0 = (FT)B.dxi[0], dxi1 = (FT)B.dxi[1], dxi2 = (FT)B.dxi[2], prefF = (FT)pref;
 
-    FT ww[9], wp[9];
+    // z windows: w and p always; in the FP32 mode also mx, my, mz, E (as the flux type),
+    // whose z taps would otherwise be 32 global loads per cell per plane that mostly
+    // miss L2 (the FP64 mode has no registers to spare for them).
+    constexpr bool CWIN = sizeof(FT) == 4;
+    FT ww[9], wp[9], wmx[9], wmy[9], wmz[9], wE[9];
 #pragma unroll
     for (int m = 0; m < 8; m++) {
         const int o = wrapc(z0 - 4 + m, e) * e2 + own;
         ww[m] = q_w[o]; wp[m] = q_p[o];
+        if (CWIN) {
+            wmx[m] = (FT)c_mx[o]; wmy[m] = (FT)c_my[o]; wmz[m] = (FT)c_mz[o]; wE[m] = (FT)c_E[o];
+        }
     }
     // prefetched: next leading plane (w, p) and next centre (u, v, conserved)
-    FT nw, np, nu, nv;
+    FT nw, np, nu, nv, lmx = 0, lmy = 0, lmz = 0, lE = 0;
     double nmx, nmy, nmz, nE;
     {
         const int o = wrapc(z0 + 4, e) * e2 + own, oc = z0 * e2 + own;
         nw = q_w[o]; np = q_p[o];
         nu = q_u[oc]; nv = q_v[oc];
         nmx = c_mx[oc]; nmy = c_my[oc]; nmz = c_mz[oc]; nE = c_E[oc];
+        if (CWIN) {
+            lmx = (FT)c_mx[o]; lmy = (FT)c_my[o]; lmz = (FT)c_mz[o]; lE = (FT)c_E[o];
+        }
     }
     unsigned long long cmax = 0ull;
     for (int k = z0; k < z0 + nz; k++) {
         const int cp = k * e2;
         const int zo = cp + own;
         ww[8] = nw; wp[8] = np;
+        if (CWIN) {
+            wmx[8] = lmx; wmy[8] = lmy; wmz[8] = lmz; wE[8] = lE;
+        }
         const FT u_own = nu, v_own = nv;
         const FT mx_own = (FT)nmx, my_own = (FT)nmy, mz_own = (FT)nmz, E_own = (FT)nE;
         const double cur_c[5] = {0.0, nmx, nmy, nmz, nE};
@@ -492,6 +505,9 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
             nw = q_w[o]; np = q_p[o];
             nu = q_u[oc]; nv = q_v[oc];
             nmx = c_mx[oc]; nmy = c_my[oc]; nmz = c_mz[oc]; nE = c_E[oc];
+            if (CWIN) {
+                lmx = (FT)c_mx[o]; lmy = (FT)c_my[o]; lmz = (FT)c_mz[o]; lE = (FT)c_E[o];
+            }
         }
         double rkd[4], rkb[5], rho_c = 0.0;
         if (t.active) {  // this cell's RK operands
@@ -530,10 +546,10 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
             F[IENE] = F[IENE] - (KHENE(YE, YV, YPF)) * dxi1;
             FENCE();
 #define ZPL(m) (wrapc(k + (m), e) * e2 + own)
-#define ZMX(m) ((FT)c_mx[ZPL(m)])
-#define ZMY(m) ((FT)c_my[ZPL(m)])
-#define ZMZ(m) ((FT)c_mz[ZPL(m)])
-#define ZE(m) ((FT)c_E[ZPL(m)])
+#define ZMX(m) (CWIN ? wmx[4 + (m)] : (FT)c_mx[ZPL(m)])
+#define ZMY(m) (CWIN ? wmy[4 + (m)] : (FT)c_my[ZPL(m)])
+#define ZMZ(m) (CWIN ? wmz[4 + (m)] : (FT)c_mz[ZPL(m)])
+#define ZE(m) (CWIN ? wE[4 + (m)] : (FT)c_E[ZPL(m)])
 #define ZW(m) ww[4 + (m)]
 #define ZP(m) wp[4 + (m)]
 #define ZPF(m) (wp[4 + (m)] + prefF)
@@ -570,6 +586,9 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
         for (int m = 0; m < 8; m++) {
             ww[m] = ww[m + 1];
             wp[m] = wp[m + 1];
+            if (CWIN) {
+                wmx[m] = wmx[m + 1]; wmy[m] = wmy[m + 1]; wmz[m] = wmz[m + 1]; wE[m] = wE[m + 1];
+            }
         }
     }
     return cmax;

@@ -102,6 +102,12 @@ def summarize(results, wall: float):
             "mean_latency_s": float(lat.mean())}
 
 
+def _progress(every: int, done: int, total: int, t: float):
+    """One line per `every` finished requests (long runs stay visibly alive)."""
+    if every and done % every == 0:
+        print(f"served {done}/{total} requests at t={t:.1f} s", flush=True)
+
+
 class _Clock:
     """Stream time: real time since start, fast-forwarded over idle gaps when the
     server has nothing to do (so a sparse stream does not waste the GPU's time
@@ -124,11 +130,12 @@ class BatchServer:
     """Continuous batching on one BatchEngine."""
 
     def __init__(self, hbm_budget: int, fp32: bool = True, max_boxes: int = 4096,
-                 max_chunk: int = 16, keep_states=()):
+                 max_chunk: int = 16, keep_states=(), progress_every: int = 0):
         from .batch import BatchEngine
         self.engine = BatchEngine(hbm_budget, fp32=fp32, max_boxes=max_boxes)
         self.max_chunk = max_chunk
         self.keep = set(keep_states)
+        self.progress_every = progress_every
 
     def run(self, requests):
         import torch
@@ -180,6 +187,8 @@ class BatchServer:
                     if req.rid in self.keep:
                         res.state = eng.read(bid).clone()
                     eng.remove(bid)
+                    _progress(self.progress_every, sum(1 for r in results.values() if r.done == r.done),
+                              len(requests), t)
         wall = max((r.done for r in results.values()), default=0.0) - min(
             (r.arrival for r in results.values()), default=0.0)
         out = [results[r.rid] for r in sorted(requests, key=lambda r: r.rid)]
@@ -193,8 +202,9 @@ class PlainServer:
     """The plain path on the same stream: one box at a time through cns_advance
     (ghosted layout, dt to the host every step), first come first served."""
 
-    def __init__(self, keep_states=()):
+    def __init__(self, keep_states=(), progress_every: int = 0):
         self.keep = set(keep_states)
+        self.progress_every = progress_every
 
     def run(self, requests):
         import torch
@@ -221,6 +231,7 @@ class PlainServer:
             st.close()
             del U, Unew, Q, D, F
             results.append(res)
+            _progress(self.progress_every, len(results), len(requests), res.done)
         wall = max(r.done for r in results) - min(r.arrival for r in results)
         out = sorted(results, key=lambda r: r.rid)
         return out, summarize(out, wall)
