@@ -452,19 +452,22 @@ int cns_advance_async(cns_ctx *ctx, double *U, double *Unew, double *Q, double *
             // conserved fields itself, and the first two stages' ctoprim stores only the
             // u, v, w, T that diffterm reads; the third stores all of Q, as the
             // reference leaves it.
+            // ctoprim also writes the wz field diffterm reads (one z-marching pass).
+            const double *dxi = ctx->dxi;
+            double *WZ = ctx->d_wz;
             TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, U, s));
             HIPCHK(launch_courno_reset(ctx->d_acc, 1.0E-50, s));
-            TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, U, Q, ctx->d_acc, ctx->dx[0], ctx->dx[1], ctx->dx[2], s, false));
+            TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim_wz(g, U, Q, WZ, ctx->d_acc, ctx->dx[0], ctx->dx[1], ctx->dx[2], dxi[2], false, s));
             HIPCHK(launch_courno_to_dt(ctx->d_acc, ctx->cfl, dt, courno_dev ? courno_dev : ctx->d_courno, s));
-            TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
+            TIMED(ctx, CNS_PROF_DIFFTERM, s, launch_diffterm_pipe(g, Q, WZ, D, dxi[0], dxi[1], dxi[2], ctx->eta, ctx->alam, s));
             TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, U, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, T, dt, 1, false, s));
             TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, T, s));
-            TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, T, Q, nullptr, 0.0, 0.0, 0.0, s, false));
-            TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
+            TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim_wz(g, T, Q, WZ, nullptr, 0.0, 0.0, 0.0, dxi[2], false, s));
+            TIMED(ctx, CNS_PROF_DIFFTERM, s, launch_diffterm_pipe(g, Q, WZ, D, dxi[0], dxi[1], dxi[2], ctx->eta, ctx->alam, s));
             TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, T, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, Unew, dt, 2, false, s));
             TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, Unew, s));
-            TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim(g, Unew, Q, nullptr, 0.0, 0.0, 0.0, s, true));
-            TIMED(ctx, CNS_PROF_DIFFTERM, s, stencil_diffterm(g, Q, D, ctx->dxi, ctx->eta, ctx->alam, ctx->d_wz, s));
+            TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim_wz(g, Unew, Q, WZ, nullptr, 0.0, 0.0, 0.0, dxi[2], true, s));
+            TIMED(ctx, CNS_PROF_DIFFTERM, s, launch_diffterm_pipe(g, Q, WZ, D, dxi[0], dxi[1], dxi[2], ctx->eta, ctx->alam, s));
             TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, Unew, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, U, dt, 3, true, s));
             return CNS_OK;
         }

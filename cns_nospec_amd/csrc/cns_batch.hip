@@ -62,12 +62,8 @@ __device__ __forceinline__ int wrapc(int a, int e) { return a < 0 ? a + e : (a >
 __device__ __forceinline__ void prim_cell(double r, double mx, double my, double mz, double E,
                                           double &u, double &v, double &w, double &p, double &T)
 {
-    const double rhoinv = 1.0E0 / r;
-    u = mx * rhoinv;
-    v = my * rhoinv;
-    w = mz * rhoinv;
-    const double eint = E * rhoinv - 0.5E0 * ((u * u) + (v * v) + (w * w));
-    p = (GAMMA - 1.0E0) * eint * r;
+    double eint;
+    ctoprim_cell(r, mx, my, mz, E, u, v, w, eint, p);
     T = eint / CV;
 }
 

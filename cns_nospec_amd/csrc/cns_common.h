@@ -81,6 +81,20 @@ __device__ __forceinline__ int64_t iidx(const Geom &g, int i, int j, int k)
 // integer atomicMax is an exact floating max.  Any NaN is stored as the
 // canonical quiet NaN 0x7ff8..., which orders above +inf and so survives the
 // max; the host turns it into an error.
+// ctoprim of one cell (cpu/ctoprim.cpp:80-100): the single definition of the
+// primitive arithmetic, shared by every kernel that derives primitives, so that all
+// of them produce the same bits.
+__device__ __forceinline__ void ctoprim_cell(double rho, double mx, double my, double mz, double E,
+                                             double &u, double &v, double &w, double &eint, double &p)
+{
+    const double rhoinv = 1.0E0 / rho;
+    u = mx * rhoinv;
+    v = my * rhoinv;
+    w = mz * rhoinv;
+    eint = E * rhoinv - 0.5E0 * ((u * u) + (v * v) + (w * w));
+    p = (GAMMA - 1.0E0) * eint * rho;
+}
+
 __device__ __forceinline__ unsigned long long courno_bits(double v)
 {
     unsigned long long b = (unsigned long long)__double_as_longlong(v);

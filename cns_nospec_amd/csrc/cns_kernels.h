@@ -61,6 +61,11 @@ hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, 
                                double dxi1, double dxi2, const double *d, const double *base,
                                double *out, const double *dt_dev, int phase, bool write_f, hipStream_t s);
 hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hipStream_t s);
+// ctoprim (as launch_ctoprim, full or u, v, w, T only) that also writes the wz field of
+// launch_wz from the same state, in one z-marching pass.
+hipError_t launch_ctoprim_wz(const Geom &g, const double *u, double *q, double *wz,
+                             unsigned long long *acc, double dx0, double dx1, double dx2, double dxi2,
+                             bool full, hipStream_t s);
 hipError_t launch_diffterm_pipe(const Geom &g, const double *q, const double *wz, double *difflux,
                                 double dxi0, double dxi1, double dxi2, double eta, double alam,
                                 hipStream_t s);

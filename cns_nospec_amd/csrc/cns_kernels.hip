@@ -113,12 +113,8 @@ ctoprim_kernel(Geom g, const double *__restrict__ u, double *__restrict__ q,
         const int i = (int)(e - row * g.xg);
         if (i >= g.g0) continue;
         const double rho = u[e];
-        const double rhoinv = 1.0E0 / rho;
-        const double qu = u[e + cg] * rhoinv;
-        const double qv = u[e + 2 * cg] * rhoinv;
-        const double qw = u[e + 3 * cg] * rhoinv;
-        const double eint = u[e + 4 * cg] * rhoinv - 0.5E0 * ((qu * qu) + (qv * qv) + (qw * qw));
-        const double p = (GAMMA - 1.0E0) * eint * rho;
+        double qu, qv, qw, eint, p;
+        ctoprim_cell(rho, u[e + cg], u[e + 2 * cg], u[e + 3 * cg], u[e + 4 * cg], qu, qv, qw, eint, p);
         if (full) q[e] = rho;
         q[e + cg] = qu;
         q[e + 2 * cg] = qv;

@@ -185,12 +185,8 @@ __device__ __forceinline__ void hyp_flux(const double *s_mx, const double *s_my,
 __device__ __forceinline__ void prim_uvwp(double rho, double mx, double my, double mz, double E,
                                           double &u, double &v, double &w, double &p)
 {
-    const double rhoinv = 1.0E0 / rho;
-    u = mx * rhoinv;
-    v = my * rhoinv;
-    w = mz * rhoinv;
-    const double eint = E * rhoinv - 0.5E0 * ((u * u) + (v * v) + (w * w));
-    p = (GAMMA - 1.0E0) * eint * rho;
+    double eint;
+    ctoprim_cell(rho, mx, my, mz, E, u, v, w, eint, p);
 }
 
 // The stage update of the cell and its stores (STAGE1: cons + dt(D+F), the reference's
@@ -430,6 +426,92 @@ hypterm_cons_kernel(Geom g, const double *__restrict__ cons, double *__restrict_
         for (int m = 0; m < 8; m++) {
             wmx[m] = wmx[m + 1]; wmy[m] = wmy[m + 1]; wmz[m] = wmz[m + 1];
             wE[m] = wE[m + 1]; ww[m] = ww[m + 1]; wp[m] = wp[m + 1];
+        }
+    }
+}
+
+// ========================================================================= ctoprim+wz
+// ctoprim over the ghosted box (cpu/ctoprim.cpp:11-105), z-marching: one thread per
+// (i, j) column of the ghosted plane and a run of kz planes.  The column's w window
+// also gives wz = D1_z(w)*dxinv(3) for the run's interior planes (cpu/diffterm.cpp:
+// 126-151, the field diffterm_pipe_kernel reads), so the step needs no separate wz
+// pass; the 4 planes on either side of the run are read for w only.  COURNO: the
+// Courant maximum over interior cells (cpu/ctoprim.cpp:42-69) into acc, one 64-bit
+// atomicMax per block.  !FULL: only u, v, w and T are stored (what diffterm reads).
+constexpr int kCtoBlock = 256;
+
+template <bool COURNO, bool FULL>
+__global__ void __launch_bounds__(kCtoBlock)
+ctoprim_wz_kernel(Geom g, const double *__restrict__ u, double *__restrict__ q, double *__restrict__ wz,
+                  unsigned long long *__restrict__ acc, double dx0, double dx1, double dx2, double dxi2,
+                  int kz)
+{
+    const int64_t pcol = (int64_t)blockIdx.x * kCtoBlock + threadIdx.x;  // j*xg + i
+    const int i = (int)(pcol % g.xg), j = (int)(pcol / g.xg);
+    const bool col = pcol < g.pg && i < g.g0;
+    const int kbeg = blockIdx.y * kz, kend = min(kbeg + kz, g.g2);
+    const int64_t cg = g.cg, pg = g.pg;
+    const bool xy_interior = i >= NG && i < g.n0 + NG && j >= NG && j < g.n1 + NG;
+    unsigned long long cmax = 0ull;
+    double w[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) w[m] = 0.0;
+    const int k0 = max(kbeg - NG, 0), k1 = min(kend + NG, g.g2);
+    for (int kk = k0; kk < k1; kk++) {
+        const int64_t e = (int64_t)kk * pg + pcol;
+        double wv = 0.0;
+        if (col) {
+            if (kk >= kbeg && kk < kend) {
+                const double rho = u[e];
+                double qu, qv, qw, eint, p;
+                ctoprim_cell(rho, u[e + cg], u[e + 2 * cg], u[e + 3 * cg], u[e + 4 * cg], qu, qv, qw, eint, p);
+                if (FULL) q[e] = rho;
+                q[e + cg] = qu;
+                q[e + 2 * cg] = qv;
+                q[e + 3 * cg] = qw;
+                if (FULL) q[e + 4 * cg] = p;
+                q[e + 5 * cg] = eint / CV;
+                wv = qw;
+                if (COURNO && xy_interior && kk >= NG && kk < g.n2 + NG) {
+                    const double c = sqrt(GAMMA * p / rho);
+                    unsigned long long b = courno_bits((c + fabs(qu)) / dx0);
+                    const unsigned long long by = courno_bits((c + fabs(qv)) / dx1);
+                    const unsigned long long bz = courno_bits((c + fabs(qw)) / dx2);
+                    b = b > by ? b : by;
+                    b = b > bz ? b : bz;
+                    cmax = cmax > b ? cmax : b;
+                }
+            } else {
+                // w of a plane beyond the run: ctoprim's w exactly
+                double qu, qv, eint, p;
+                ctoprim_cell(u[e], u[e + cg], u[e + 2 * cg], u[e + 3 * cg], u[e + 4 * cg], qu, qv, wv, eint, p);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) w[m] = w[m + 1];
+        w[8] = wv;
+        // the window now holds planes kk-8 .. kk; its centre c = kk-4 is complete when
+        // c-4 >= k0, which holds for every interior c of the run
+        const int c = kk - NG;
+        if (col && c >= kbeg && c < kend && c >= NG && c < g.n2 + NG) {
+#define WW(mm) w[4 + (mm)]
+            wz[(int64_t)c * pg + pcol] = D1(WW) * dxi2;
+#undef WW
+        }
+    }
+    if (COURNO) {
+        __shared__ unsigned long long wmax[kCtoBlock / 64];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(cmax, off);
+            cmax = cmax > o ? cmax : o;
+        }
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = cmax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long m = wmax[0];
+            for (int wv = 1; wv < kCtoBlock / 64; wv++) m = m > wmax[wv] ? m : wmax[wv];
+            atomicMax(acc, m);
         }
     }
 }
@@ -729,6 +811,21 @@ hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, 
     else
         write_f ? HYP_CONS(false, true) : HYP_CONS(false, false);
 #undef HYP_CONS
+    return hipGetLastError();
+}
+
+hipError_t launch_ctoprim_wz(const Geom &g, const double *u, double *q, double *wz,
+                             unsigned long long *acc, double dx0, double dx1, double dx2, double dxi2,
+                             bool full, hipStream_t s)
+{
+    const int kz = 64;
+    const dim3 grid((unsigned)((g.pg + kCtoBlock - 1) / kCtoBlock), (unsigned)((g.g2 + kz - 1) / kz));
+#define CTO_WZ(C, F) ctoprim_wz_kernel<C, F><<<grid, kCtoBlock, 0, s>>>(g, u, q, wz, acc, dx0, dx1, dx2, dxi2, kz)
+    if (acc)
+        full ? CTO_WZ(true, true) : CTO_WZ(true, false);
+    else
+        full ? CTO_WZ(false, true) : CTO_WZ(false, false);
+#undef CTO_WZ
     return hipGetLastError();
 }
 
