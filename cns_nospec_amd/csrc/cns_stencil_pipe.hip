@@ -580,25 +580,29 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
 
     // windows: plane k+m at [4+m]; ux and vy of the first 8 planes straight from
     // global memory (the same expressions as the per-plane path)
+    // The tile's own-row ux and own-cell vy feed phase B's LDS planes (uxy, vyx of the
+    // neighbours), so ux is needed on every x-interior lane of the ghosted y range and
+    // vy on every y-interior lane of the ghosted x range; both stencils stay in bounds.
     double wu[9], wv[9], ww[9], wt[9], wux[9], wvy[9];
     double nw = 0.0, nt = 0.0;
-    if (active) {
+    const bool need_ux = gi < g.n0 + NG && gj < g.g1, need_vy = gj < g.n1 + NG && gi < g.g0;
 #pragma unroll 1
-        for (int m = 0; m < 8; m++) {
-            const int64_t o = (int64_t)(kbeg + m) * pg + own;
+    for (int m = 0; m < 8; m++) {
+        const int64_t o = (int64_t)(kbeg + m) * pg + own;
+        double ux_ = 0.0, vy_ = 0.0;
 #define GUX(mm) q_u[o + (mm)]
 #define GVY(mm) q_v[o + (int64_t)(mm) * xg]
-            const double ux_ = D1(GUX) * dxi0, vy_ = D1(GVY) * dxi1;
+        if (need_ux) ux_ = D1(GUX) * dxi0;
+        if (need_vy) vy_ = D1(GVY) * dxi1;
 #undef GUX
 #undef GVY
 #pragma unroll
-            for (int mm = 0; mm < 7; mm++) {
-                wux[mm] = wux[mm + 1];
-                wvy[mm] = wvy[mm + 1];
-            }
-            wux[7] = ux_;
-            wvy[7] = vy_;
+        for (int mm = 0; mm < 7; mm++) {
+            wux[mm] = wux[mm + 1];
+            wvy[mm] = wvy[mm + 1];
         }
+        wux[7] = ux_;
+        wvy[7] = vy_;
     }
     if (inbox) {
 #pragma unroll
@@ -640,19 +644,24 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
             wvy[8] = D1(LVY) * dxi1;
 #undef LUX
 #undef LVY
-            // centre plane: ux on rows 0..15, columns 4..67 (x derivative of u)
-#pragma unroll
-            for (int h = threadIdx.x; h < R * TX; h += NT) {
-                const int l = (h >> 6) * PX + (h & 63) + 4;
+            // centre plane, ux: the own rows from the window (ux of plane k, computed from
+            // the same values four planes ago), the halo rows 0..3 and 12..15 from the tile
+            s_ux[me64] = wux[4];
+            {
+                const int rr = threadIdx.x >> 6, r = rr < 4 ? rr : rr + BY, c = threadIdx.x & 63;
+                const int l = r * PX + c + 4;
 #define CUX(mm) s_cu[l + (mm)]
-                s_ux[h] = D1(CUX) * dxi0;
+                s_ux[r * TX + c] = D1(CUX) * dxi0;
 #undef CUX
             }
-            // centre plane: vy on rows 4..11, columns 0..71 (y derivative of v)
-            for (int h = threadIdx.x; h < BY * PX; h += NT) {
-                const int l = h + 4 * PX;
+            // centre plane, vy: the own cells from the window, the x-halo columns 0..3 and
+            // 68..71 of rows 4..11 from the tile
+            s_vy[meby] = wvy[4];
+            if (threadIdx.x < 8 * BY) {
+                const int rr = threadIdx.x >> 3, cc = threadIdx.x & 7, c = cc < 4 ? cc : cc + TX;
+                const int l = (rr + 4) * PX + c;
 #define CVY(mm) s_cv[l + (mm) * PX]
-                s_vy[h] = D1(CVY) * dxi1;
+                s_vy[rr * PX + c] = D1(CVY) * dxi1;
 #undef CVY
             }
         }
