@@ -52,8 +52,10 @@ def algorithmic_bytes(kind: str, n: int, stage: int = 2) -> float:
         "rk": rk,
         # the fused pass, averaged over a step's three stages: cons and q over their halos,
         # the cell's rho and D without its zero rho row (5c), the stage base in stages 2-3
-        # (10c / 3), the result (5c) and F in stage 3 only (5c / 3)
-        "hypterm_rk": 8 * g3 + 15 * n3,
+        # (10c / 3), the result (5c), F in stage 3 only (5c / 3), and the primitives of
+        # the result stored for the next stage (u, v, w, T after stage 1, all six after
+        # stage 2: 10c / 3)
+        "hypterm_rk": 8 * g3 + 15 * n3 + 10 * n3 / 3,
         "fill_ctoprim": 10 * (g3 - n3) + 11 * g3,
     }
     return 8.0 * table[kind]
@@ -334,6 +336,38 @@ def kernels_once(edge: int, reps: int = 3):
     print(json.dumps({"kernels_once": edge, "reps": reps, "copy_bytes_each_way": src.numel() * 8}))
 
 
+def batched_once(nreq: int = 200, steps: int = 4, seed: int = 2024):
+    """The served request mix (edge 24-128) all resident in the batched engine, FP32
+    flux then FP64: one captured warm-up step and `steps` timed steps each.  Used as
+    the target of rocprofv3 runs of the batched kernels; ends with three copies of a
+    1 GiB buffer for calibrating byte counters."""
+    import torch
+    from cns_nospec_amd.batch import BatchEngine, box_bytes
+    from cns_nospec_amd.serve import make_stream, request_state
+    reqs = make_stream(nreq, 0.0, seed)
+    cells = sum(r.edge ** 3 for r in reqs)
+    for fp32 in (True, False):
+        eng = BatchEngine(sum(box_bytes(r.edge, fp32) for r in reqs) + (256 << 20), fp32=fp32,
+                          max_boxes=nreq)
+        for r in reqs:
+            eng.add(request_state(r).reshape(-1), r.edge)
+        eng.step(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.step(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        print(json.dumps({"batched_once": nreq, "fp32_flux": fp32, "cells": cells, "steps": steps,
+                          "cell_steps_per_s": cells * steps / el}), flush=True)
+        eng.close()
+        torch.cuda.empty_cache()
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)  # 1 GiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -346,6 +380,8 @@ def main():
                     help="requests of the served workload (0 skips it)")
     ap.add_argument("--kernels-once", action="store_true",
                     help="run each step kernel a few times and a calibration copy (for rocprofv3)")
+    ap.add_argument("--batched-once", action="store_true",
+                    help="a few steps of the batched engines on the request mix (for rocprofv3)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -362,6 +398,9 @@ def main():
     _lib.load()
     if args.kernels_once:
         kernels_once(args.edge)
+        return
+    if args.batched_once:
+        batched_once(args.serve_requests or 200)
         return
 
     main_run = run_plain(args.edge, args.steps, args.warmup)

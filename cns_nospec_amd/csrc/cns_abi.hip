@@ -135,6 +135,14 @@ bool pipe_on(std::initializer_list<const void *> ptrs)
     return true;
 }
 
+// The pipelined fused step stores each stage's primitives from the stage kernel
+// (no ctoprim pass in stages 2 and 3); CNS_STAGE_Q=0 selects a ctoprim pass per stage.
+bool stage_q_on()
+{
+    const char *e = getenv("CNS_STAGE_Q");
+    return !(e && atoi(e) == 0);
+}
+
 hipError_t stencil_hypterm(const Geom &g, const double *cons, const double *q, double *flux,
                            const double dxi[3], hipStream_t s)
 {
@@ -455,20 +463,42 @@ int cns_advance_async(cns_ctx *ctx, double *U, double *Unew, double *Q, double *
             // ctoprim also writes the wz field diffterm reads (one z-marching pass).
             const double *dxi = ctx->dxi;
             double *WZ = ctx->d_wz;
+            if (stage_q_on()) {
+                // Stages 1 and 2 also store ctoprim of their result (interior cells) into
+                // Q, so stages 2 and 3 skip the ctoprim pass: the periodic fill copies
+                // those cells' Q into the ghosts with the state's, and a wz pass reads w.
+                // diffterm's rho row (0.0) is stored by stage 3 only; nothing reads it.
+                const unsigned uvwt = (1u << QU) | (1u << QV) | (1u << QW) | (1u << QTEMP);
+                TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, U, s));
+                HIPCHK(launch_courno_reset(ctx->d_acc, 1.0E-50, s));
+                TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim_wz(g, U, Q, WZ, ctx->d_acc, ctx->dx[0], ctx->dx[1], ctx->dx[2], dxi[2], false, s));
+                HIPCHK(launch_courno_to_dt(ctx->d_acc, ctx->cfl, dt, courno_dev ? courno_dev : ctx->d_courno, s));
+                TIMED(ctx, CNS_PROF_DIFFTERM, s, launch_diffterm_pipe(g, Q, WZ, D, dxi[0], dxi[1], dxi[2], ctx->eta, ctx->alam, s, false));
+                TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, U, F, dxi[0], dxi[1], dxi[2], D, U, T, dt, 1, false, Q, 1, s));
+                TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary_q(g, T, Q, uvwt, s));
+                TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_wz(g, Q, WZ, dxi[2], s));
+                TIMED(ctx, CNS_PROF_DIFFTERM, s, launch_diffterm_pipe(g, Q, WZ, D, dxi[0], dxi[1], dxi[2], ctx->eta, ctx->alam, s, false));
+                TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, T, F, dxi[0], dxi[1], dxi[2], D, U, Unew, dt, 2, false, Q, 2, s));
+                TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary_q(g, Unew, Q, (1u << NQ) - 1u, s));
+                TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_wz(g, Q, WZ, dxi[2], s));
+                TIMED(ctx, CNS_PROF_DIFFTERM, s, launch_diffterm_pipe(g, Q, WZ, D, dxi[0], dxi[1], dxi[2], ctx->eta, ctx->alam, s, true));
+                TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, Unew, F, dxi[0], dxi[1], dxi[2], D, U, U, dt, 3, true, nullptr, 0, s));
+                return CNS_OK;
+            }
             TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, U, s));
             HIPCHK(launch_courno_reset(ctx->d_acc, 1.0E-50, s));
             TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim_wz(g, U, Q, WZ, ctx->d_acc, ctx->dx[0], ctx->dx[1], ctx->dx[2], dxi[2], false, s));
             HIPCHK(launch_courno_to_dt(ctx->d_acc, ctx->cfl, dt, courno_dev ? courno_dev : ctx->d_courno, s));
             TIMED(ctx, CNS_PROF_DIFFTERM, s, launch_diffterm_pipe(g, Q, WZ, D, dxi[0], dxi[1], dxi[2], ctx->eta, ctx->alam, s));
-            TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, U, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, T, dt, 1, false, s));
+            TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, U, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, T, dt, 1, false, nullptr, 0, s));
             TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, T, s));
             TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim_wz(g, T, Q, WZ, nullptr, 0.0, 0.0, 0.0, dxi[2], false, s));
             TIMED(ctx, CNS_PROF_DIFFTERM, s, launch_diffterm_pipe(g, Q, WZ, D, dxi[0], dxi[1], dxi[2], ctx->eta, ctx->alam, s));
-            TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, T, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, Unew, dt, 2, false, s));
+            TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, T, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, Unew, dt, 2, false, nullptr, 0, s));
             TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, Unew, s));
             TIMED(ctx, CNS_PROF_CTOPRIM, s, launch_ctoprim_wz(g, Unew, Q, WZ, nullptr, 0.0, 0.0, 0.0, dxi[2], true, s));
             TIMED(ctx, CNS_PROF_DIFFTERM, s, launch_diffterm_pipe(g, Q, WZ, D, dxi[0], dxi[1], dxi[2], ctx->eta, ctx->alam, s));
-            TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, Unew, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, U, dt, 3, true, s));
+            TIMED(ctx, CNS_PROF_HYPTERM_RK, s, launch_hypterm_cons(g, Unew, F, ctx->dxi[0], ctx->dxi[1], ctx->dxi[2], D, U, U, dt, 3, true, nullptr, 0, s));
             return CNS_OK;
         }
         TIMED(ctx, CNS_PROF_FILL, s, launch_fill_boundary(g, U, s));

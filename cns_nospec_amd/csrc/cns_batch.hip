@@ -656,16 +656,23 @@ hipError_t stage_class(BoxDev *boxes, const TileDev *tiles, int n, int grid, int
     return hipGetLastError();
 }
 
-template <typename FT, int LTX>
-int class_occupancy()
+// Blocks per CU of one kernel (at least 1).
+template <typename KFN>
+int occupancy(KFN kfn)
 {
-    int pd = 1, ph = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pd, batch_diff_kernel<FT, LTX>, kBatchThreads, 0) != hipSuccess)
-        pd = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&ph, batch_hyp_kernel<FT, LTX>, kBatchThreads, 0) != hipSuccess)
-        ph = 1;
-    const int per = pd < ph ? pd : ph;
+    int per = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, kBatchThreads, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        per = 1;
+    }
     return per < 1 ? 1 : per;
+}
+
+template <typename FT, int LTX>
+void class_grids(int cus, int &gd, int &gh)
+{
+    gd = cus * occupancy(batch_diff_kernel<FT, LTX>);
+    gh = cus * occupancy(batch_hyp_kernel<FT, LTX>);
 }
 
 }  // namespace
@@ -690,14 +697,15 @@ hipError_t launch_batch_finalize(BoxDev *boxes, const int *ids, int n, int count
 }
 
 hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int start[3],
-                              const int count[3], const int grid[3], int stage, bool fp32, hipStream_t s)
+                              const int count[3], const int grid[6], int stage, bool fp32, hipStream_t s)
 {
     for (int pass = 0; pass < 2; pass++) {  // all diffterm launches, then all hypterm launches
         const bool diff = pass == 0;
         for (int c = 0; c < 3; c++) {
             const int n = count[c];
             if (n <= 0) continue;
-            const int g = grid[c] < n ? grid[c] : n;
+            const int gc = grid[pass * 3 + c];
+            const int g = gc < n ? gc : n;
             const TileDev *t = tiles + start[c];
             hipError_t e;
             if (fp32)
@@ -714,7 +722,7 @@ hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int sta
     return hipSuccess;
 }
 
-void batch_class_grids(bool fp32, int grid[3])
+void batch_class_grids(bool fp32, int grid[6])
 {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
@@ -722,13 +730,13 @@ void batch_class_grids(bool fp32, int grid[3])
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
     }
     if (fp32) {
-        grid[0] = cus * class_occupancy<float, 4>();
-        grid[1] = cus * class_occupancy<float, 5>();
-        grid[2] = cus * class_occupancy<float, 6>();
+        class_grids<float, 4>(cus, grid[0], grid[3]);
+        class_grids<float, 5>(cus, grid[1], grid[4]);
+        class_grids<float, 6>(cus, grid[2], grid[5]);
     } else {
-        grid[0] = cus * class_occupancy<double, 4>();
-        grid[1] = cus * class_occupancy<double, 5>();
-        grid[2] = cus * class_occupancy<double, 6>();
+        class_grids<double, 4>(cus, grid[0], grid[3]);
+        class_grids<double, 5>(cus, grid[1], grid[4]);
+        class_grids<double, 6>(cus, grid[2], grid[5]);
     }
 }
 

@@ -47,10 +47,12 @@ hipError_t launch_batch_init(BoxDev *boxes, int box, int e, bool fp32, hipStream
 hipError_t launch_batch_finalize(BoxDev *boxes, const int *box_ids, int nbox, int count_step,
                                  hipStream_t s);
 // One RK stage over the tile list: the list holds class 0, 1, 2 tiles in the
-// ranges [start[c], start[c]+count[c]); grid[c] caps each persistent launch.
+// ranges [start[c], start[c]+count[c]); grid[c] caps class c's persistent diffterm
+// launch and grid[3 + c] its hypterm launch.
 hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int start[3],
-                              const int count[3], const int grid[3], int stage, bool fp32, hipStream_t s);
-// Resident-workgroup capacity of each class's kernels (CUs x blocks per CU).
-void batch_class_grids(bool fp32, int grid[3]);
+                              const int count[3], const int grid[6], int stage, bool fp32, hipStream_t s);
+// Resident-workgroup capacity (CUs x blocks per CU) of each class's diffterm kernel
+// (grid[c]) and hypterm kernel (grid[3 + c]).
+void batch_class_grids(bool fp32, int grid[6]);
 
 }  // namespace cns

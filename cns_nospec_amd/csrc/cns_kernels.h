@@ -7,6 +7,9 @@ namespace cns {
 
 // Ghost shell, closed-form periodic image (cns_kernels.hip).
 hipError_t launch_fill_boundary(const Geom &g, double *u, hipStream_t s);
+// The same periodic fill of u's NC components and of the components of q selected by
+// qmask (bit l for component l), in one pass.
+hipError_t launch_fill_boundary_q(const Geom &g, double *u, double *q, unsigned qmask, hipStream_t s);
 
 // Resets a Courant accumulator slot to the bits of max(init, 0) (cns_kernels.hip).
 hipError_t launch_courno_reset(unsigned long long *acc, double init, hipStream_t s);
@@ -57,17 +60,22 @@ hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *
                                bool write_f, hipStream_t s);
 // The same fused stage reading only cons (phases 1..3): u, v, w, p are derived from it
 // with ctoprim's arithmetic, so it equals launch_hypterm_pipe with q = ctoprim(cons).
+// qout (with write_f false): ctoprim of the result on interior cells into qout, u, v,
+// w and T (qout_kind 1) or all six components (2); 0 or a null qout stores none.
 hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, double dxi0,
                                double dxi1, double dxi2, const double *d, const double *base,
-                               double *out, const double *dt_dev, int phase, bool write_f, hipStream_t s);
+                               double *out, const double *dt_dev, int phase, bool write_f,
+                               double *qout, int qout_kind, hipStream_t s);
 hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hipStream_t s);
 // ctoprim (as launch_ctoprim, full or u, v, w, T only) that also writes the wz field of
 // launch_wz from the same state, in one z-marching pass.
 hipError_t launch_ctoprim_wz(const Geom &g, const double *u, double *q, double *wz,
                              unsigned long long *acc, double dx0, double dx1, double dx2, double dxi2,
                              bool full, hipStream_t s);
+// write_rho = false leaves difflux's rho row (always 0.0) unwritten, for a caller that
+// stores it once and never reads it in between.
 hipError_t launch_diffterm_pipe(const Geom &g, const double *q, const double *wz, double *difflux,
                                 double dxi0, double dxi1, double dxi2, double eta, double alam,
-                                hipStream_t s);
+                                hipStream_t s, bool write_rho = true);
 
 }  // namespace cns

@@ -36,7 +36,8 @@ __device__ __forceinline__ int wrap_ghost(int a, int n)
 // values.  The shell is enumerated compactly: first the full rows whose j or k is
 // a ghost index (g0 cells each), then the 2*NG ghost cells of each interior row.
 __global__ void __launch_bounds__(kStreamBlock)
-fill_boundary_kernel(Geom g, double *__restrict__ u, int64_t n_rows_a, int64_t total)
+fill_boundary_kernel(Geom g, double *__restrict__ u, double *__restrict__ q, unsigned qmask,
+                     int64_t n_rows_a, int64_t total)
 {
     const int64_t plane_rows = (int64_t)2 * NG * g.g1;
     const int64_t elems_a = n_rows_a * g.g0;
@@ -70,6 +71,9 @@ fill_boundary_kernel(Geom g, double *__restrict__ u, int64_t n_rows_a, int64_t t
         const int64_t src = gidx(g, wrap_ghost(i, g.n0), wrap_ghost(j, g.n1), wrap_ghost(k, g.n2));
 #pragma unroll
         for (int l = 0; l < NC; l++) u[l * g.cg + dst] = u[l * g.cg + src];
+#pragma unroll
+        for (int l = 0; l < NQ; l++)
+            if (qmask & (1u << l)) q[l * g.cg + dst] = q[l * g.cg + src];
     }
 }
 
@@ -342,7 +346,16 @@ hipError_t launch_fill_boundary(const Geom &g, double *u, hipStream_t s)
 {
     const int64_t rows_a = (int64_t)2 * NG * g.g1 + (int64_t)g.n2 * 2 * NG;
     const int64_t total = rows_a * g.g0 + (int64_t)g.n1 * g.n2 * 2 * NG;
-    fill_boundary_kernel<<<stream_grid(total, 2), kStreamBlock, 0, s>>>(g, u, rows_a, total);
+    fill_boundary_kernel<<<stream_grid(total, 2), kStreamBlock, 0, s>>>(g, u, nullptr, 0u, rows_a, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_boundary_q(const Geom &g, double *u, double *q, unsigned qmask, hipStream_t s)
+{
+    if (!q) qmask = 0;
+    const int64_t rows_a = (int64_t)2 * NG * g.g1 + (int64_t)g.n2 * 2 * NG;
+    const int64_t total = rows_a * g.g0 + (int64_t)g.n1 * g.n2 * 2 * NG;
+    fill_boundary_kernel<<<stream_grid(total, 2), kStreamBlock, 0, s>>>(g, u, q, qmask & 63u, rows_a, total);
     return hipGetLastError();
 }
 

@@ -110,6 +110,7 @@ struct PipeRk {
     double *out;         // result (ghosted); must not alias cons
     const double *dt;    // device dt
     int phase;           // 0: plain hypterm
+    double *qout;        // primitives of the result (interior cells), or null
 };
 
 // hypterm's flux of the lane's cell (LDS index me of 16 x 72 tiles): x terms from the
@@ -196,15 +197,35 @@ template <bool STAGE1, bool WRITE_F>
 __device__ __forceinline__ void rk_store(const double (&F)[5], const double (&uc)[5], const double (&rkd)[5],
                                          const double (&rkb)[5], double ra, double rb, double dt,
                                          double *flux, int64_t fo, int64_t c, double *out, int64_t zo,
-                                         int64_t cg)
+                                         int64_t cg, double (&o)[5])
 {
     if (WRITE_F) {
 #pragma unroll
         for (int l = 0; l < NC; l++) flux[l * c + fo] = F[l];
     }
 #pragma unroll
-    for (int l = 0; l < NC; l++)
-        out[l * cg + zo] = STAGE1 ? uc[l] + dt * (rkd[l] + F[l]) : ra * rkb[l] + rb * (uc[l] + dt * (rkd[l] + F[l]));
+    for (int l = 0; l < NC; l++) {
+        o[l] = STAGE1 ? uc[l] + dt * (rkd[l] + F[l]) : ra * rkb[l] + rb * (uc[l] + dt * (rkd[l] + F[l]));
+        out[l * cg + zo] = o[l];
+    }
+}
+
+// ctoprim of the stage result o at interior cell zo (cpu/ctoprim.cpp:23-38), stored
+// into q: u, v, w and T (QOUT 1: what the next stage's diffterm reads) or every
+// component (QOUT 2: the Q the reference leaves after a step).  The ghost cells are
+// the periodic fill's, so q over the ghosted box equals ctoprim of the filled result.
+template <int QOUT>
+__device__ __forceinline__ void q_store(const double (&o)[5], double *q, int64_t zo, int64_t cg)
+{
+    if (QOUT == 0) return;
+    double u, v, w, eint, p;
+    ctoprim_cell(o[IRHO], o[IMX], o[IMY], o[IMZ], o[IENE], u, v, w, eint, p);
+    if (QOUT == 2) q[QRHO * cg + zo] = o[IRHO];
+    q[QU * cg + zo] = u;
+    q[QV * cg + zo] = v;
+    q[QW * cg + zo] = w;
+    if (QOUT == 2) q[QPRES * cg + zo] = p;
+    q[QTEMP * cg + zo] = eint / CV;
 }
 
 // ======================================================================= hypterm
@@ -299,7 +320,8 @@ hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__res
             hyp_flux(s_mx, s_my, s_mz, s_E, s_p, s_u, s_v, me, wmx, wmy, wmz, wE, ww, wp, dxi0, dxi1, dxi2, F);
             if (RK) {
                 const double uc[5] = {rho_c, wmx[4], wmy[4], wmz[4], wE[4]};
-                rk_store<STAGE1, WRITE_F>(F, uc, rkd, rkb, ra, rb, dt, flux, fo, g.c, rk.out, zo, cg);
+                double o[5];
+                rk_store<STAGE1, WRITE_F>(F, uc, rkd, rkb, ra, rb, dt, flux, fo, g.c, rk.out, zo, cg, o);
             } else {
 #pragma unroll
                 for (int l = 0; l < NC; l++) flux[l * g.c + fo] = F[l];
@@ -325,7 +347,7 @@ enum { SDU = 0, SDV, SDP, SDNF };
 constexpr int XSTRIP = BY * PX;             // rows 4..11, all 72 columns
 constexpr int CROSS = XSTRIP + 8 * TX;      // + rows 0..3 and 12..15, columns 4..67
 
-template <bool STAGE1, bool WRITE_F>
+template <bool STAGE1, bool WRITE_F, int QOUT>
 __global__ void __launch_bounds__(NT, 2)
 hypterm_cons_kernel(Geom g, const double *__restrict__ cons, double *__restrict__ flux, double dxi0,
                     double dxi1, double dxi2, int kz, int ntx, int nty, PipeRk rk)
@@ -420,7 +442,9 @@ hypterm_cons_kernel(Geom g, const double *__restrict__ cons, double *__restrict_
             hyp_flux(b + CMX * TILE, b + CMY * TILE, b + CMZ * TILE, b + CE * TILE, sd + SDP * TILE,
                      sd + SDU * TILE, sd + SDV * TILE, me, wmx, wmy, wmz, wE, ww, wp, dxi0, dxi1, dxi2, F);
             const double uc[5] = {b[CRHO * TILE + me], wmx[4], wmy[4], wmz[4], wE[4]};
-            rk_store<STAGE1, WRITE_F>(F, uc, rkd, rkb, ra, rb, dt, flux, fo, g.c, rk.out, zo, cg);
+            double o[5];
+            rk_store<STAGE1, WRITE_F>(F, uc, rkd, rkb, ra, rb, dt, flux, fo, g.c, rk.out, zo, cg, o);
+            q_store<QOUT>(o, rk.qout, zo, cg);
         }
 #pragma unroll
         for (int m = 0; m < 8; m++) {
@@ -550,6 +574,7 @@ wz_kernel(Geom g, const double *__restrict__ q_w, double *__restrict__ wz, doubl
 // Register windows: u, v, w, T and the computed ux, vy.
 enum { DCU = 0, DCV, DCW, DCT, DWZ, DLU, DLV, DNF };
 
+template <bool WRITE_RHO>
 __global__ void __launch_bounds__(NT, 2)
 diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restrict__ wzf,
                      double *__restrict__ difflux, double dxi0, double dxi1, double dxi2, double eta,
@@ -730,7 +755,7 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
             mechwork = eta * mechwork + dmx * wu[4] + dmy * wv[4] + dmz * ww[4];
             const int64_t fo = iidx(g, gi - NG, gj - NG, k);
             const int64_t c = g.c;
-            difflux[fo] = 0.0E0;
+            if (WRITE_RHO) difflux[fo] = 0.0E0;
             difflux[fo + IMX * c] = dmx;
             difflux[fo + IMY * c] = dmy;
             difflux[fo + IMZ * c] = dmz;
@@ -806,19 +831,28 @@ hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *
 
 hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, double dxi0,
                                double dxi1, double dxi2, const double *d, const double *base,
-                               double *out, const double *dt_dev, int phase, bool write_f, hipStream_t s)
+                               double *out, const double *dt_dev, int phase, bool write_f,
+                               double *qout, int qout_kind, hipStream_t s)
 {
     const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
     const int kz = pipe_kz(g, ntx * nty);
     const int nb = ntx * nty * ((g.n2 + kz - 1) / kz);
-    PipeRk rk{d, base, out, dt_dev, phase};
+    PipeRk rk{d, base, out, dt_dev, phase, qout};
     const bool stage1 = phase == 1 && base == cons;
-#define HYP_CONS(S1, WF) \
-    hypterm_cons_kernel<S1, WF><<<nb, NT, 0, s>>>(g, cons, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk)
-    if (stage1)
-        write_f ? HYP_CONS(true, true) : HYP_CONS(true, false);
+    if (!qout) qout_kind = 0;
+    if (qout_kind < 0 || qout_kind > 2 || (write_f && qout_kind)) return hipErrorInvalidValue;
+#define HYP_CONS(S1, WF, QO) \
+    hypterm_cons_kernel<S1, WF, QO><<<nb, NT, 0, s>>>(g, cons, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk)
+    // the step's combinations: stage 1 (-> q 1), stage 2 (-> q 2), stage 3 (F, no q);
+    // and no q for any stage
+    if (stage1 && !write_f)
+        qout_kind == 1 ? HYP_CONS(true, false, 1) : (qout_kind == 2 ? HYP_CONS(true, false, 2) : HYP_CONS(true, false, 0));
+    else if (stage1)
+        HYP_CONS(true, true, 0);
+    else if (!write_f)
+        qout_kind == 1 ? HYP_CONS(false, false, 1) : (qout_kind == 2 ? HYP_CONS(false, false, 2) : HYP_CONS(false, false, 0));
     else
-        write_f ? HYP_CONS(false, true) : HYP_CONS(false, false);
+        HYP_CONS(false, true, 0);
 #undef HYP_CONS
     return hipGetLastError();
 }
@@ -848,12 +882,15 @@ hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hi
 
 hipError_t launch_diffterm_pipe(const Geom &g, const double *q, const double *wz, double *difflux,
                                 double dxi0, double dxi1, double dxi2, double eta, double alam,
-                                hipStream_t s)
+                                hipStream_t s, bool write_rho)
 {
     const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
     const int kz = pipe_kz(g, ntx * nty);
     const int nb = ntx * nty * ((g.n2 + kz - 1) / kz);
-    diffterm_pipe_kernel<<<nb, NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty);
+    if (write_rho)
+        diffterm_pipe_kernel<true><<<nb, NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty);
+    else
+        diffterm_pipe_kernel<false><<<nb, NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty);
     return hipGetLastError();
 }
 

@@ -159,19 +159,22 @@ def test_64_cube_100_steps_every_step(orc, device):
 @pytest.mark.parametrize("n", [(70, 33, 40), (24, 24, 24)])
 def test_step_compositions_agree(device, monkeypatch, n):
     """Every way the library composes a step leaves every ABI array bit-identical:
-    pipelined kernels fused (hypterm + stage update in one pass) and unfused,
+    pipelined kernels fused (hypterm + stage update in one pass, with the stage's
+    primitives stored by that pass or by a ctoprim pass per stage) and unfused,
     register-staged kernels fused and unfused, and the direct kernels.
     Non-cubic boxes go beyond the reference cpu/ path (cubic only), so the direct
     one-thread-per-cell kernels are the cross-check here."""
     lay = BoxLayout(n)
     U0 = sinusoid_state(lay, 9, mach=0.2, kmax=3)
     results = {}
-    for name, pipe, fused, impl in (("pipe_fused", "1", True, _lib.CNS_IMPL_TILED),
-                                    ("pipe_unfused", "1", False, _lib.CNS_IMPL_TILED),
-                                    ("staged_fused", "0", True, _lib.CNS_IMPL_TILED),
-                                    ("staged_unfused", "0", False, _lib.CNS_IMPL_TILED),
-                                    ("direct", "1", False, _lib.CNS_IMPL_DIRECT)):
+    for name, pipe, fused, impl, stage_q in (("pipe_fused", "1", True, _lib.CNS_IMPL_TILED, "1"),
+                                             ("pipe_fused_ctoprim", "1", True, _lib.CNS_IMPL_TILED, "0"),
+                                             ("pipe_unfused", "1", False, _lib.CNS_IMPL_TILED, "1"),
+                                             ("staged_fused", "0", True, _lib.CNS_IMPL_TILED, "1"),
+                                             ("staged_unfused", "0", False, _lib.CNS_IMPL_TILED, "1"),
+                                             ("direct", "1", False, _lib.CNS_IMPL_DIRECT, "1")):
         monkeypatch.setenv("CNS_PIPE", pipe)
+        monkeypatch.setenv("CNS_STAGE_Q", stage_q)
         ops.set_stencil_impl(impl)
         try:
             st = ops.Stepper(lay)
