@@ -312,6 +312,44 @@ def served_secondary(nreq: int, seed: int = 2024):
             "error_sample": f"requests {list(keep)} of the burst run, final state"}
 
 
+def domain_secondary(edge: int = 256, m: int = 64, steps: int = 10, warmup: int = 2):
+    """A periodic edge^3 domain cut into m^3 boxes (the reference Fortran driver's
+    max_grid_size split), stepped as one multi-box domain and as one box; the dt
+    sequences must agree exactly (the states are bit-identical, as the tests check)."""
+    import torch
+    from cns_nospec_amd import ops
+    from cns_nospec_amd.domain import Domain
+    from cns_nospec_amd.layout import NC, BoxLayout
+    from cns_nospec_amd.state import device_sinusoid_state
+
+    dev = torch.device("cuda:0")
+    lay = BoxLayout.cube(edge)
+    U, Unew, Q, D, F = ops.allocate_variables(lay, dev)
+    device_sinusoid_state(lay, 2024, dev, kmax=4, out=U)
+    g, ng = lay.dimg, lay.ng
+    whole = U.view(NC, g[2], g[1], lay.xg)[:, ng:ng + edge, ng:ng + edge, ng:ng + edge]
+    dom = Domain(edge, m)
+    dU, dUnew, dQ, dD, dF = dom.allocate(dev)
+    dom.scatter(whole, NC, dU)
+    st = ops.Stepper(lay)
+    res = {}
+    for name, step in (("domain", lambda: dom.advance(dU, dUnew, dQ, dD, dF)),
+                       ("single_box", lambda: st.new_advance(U, Unew, Q, D, F))):
+        dts = [step() for _ in range(warmup)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dts += [step() for _ in range(steps)]
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[name] = {"cell_steps_per_s": edge ** 3 * steps / el, "ms_per_step": el / steps * 1e3, "dts": dts}
+    same = res["domain"].pop("dts") == res["single_box"].pop("dts")
+    st.close()
+    dom.close()
+    return {"workload": f"periodic {edge}^3 domain, max_grid_size {m} ({dom.nbox} boxes), seeded "
+                        f"sinusoids, {steps} RK3 steps", "unit": "cell-steps/s", **res,
+            "dt_sequence_identical": same}
+
+
 def kernels_once(edge: int, reps: int = 3):
     """One warm-up and `reps` launches of every kernel of the fused plain step on an
     edge^3 box, plus a plain device copy of known size (for calibrating rocprofv3
@@ -447,6 +485,7 @@ def main():
                            "vs_c2050_published": r64["cell_steps_per_s"] / C2050_64_CELL_STEPS}
         sec["batched"] = batched_secondary()
         sec["fp32_flux_error_100_steps"] = fp32_error_secondary()
+        sec["domain"] = domain_secondary()
         if args.serve_requests > 0:
             sec["served"] = served_secondary(args.serve_requests)
         out["secondary"] = sec

@@ -12,7 +12,7 @@ import re
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libcns_hip.so")
 INCLUDE = os.path.join(os.path.dirname(PKG), "include")
-HEADERS = [os.path.join(INCLUDE, h) for h in ("cns.h", "cns_batch.h", "cns_io.h")]
+HEADERS = [os.path.join(INCLUDE, h) for h in ("cns.h", "cns_batch.h", "cns_io.h", "cns_domain.h")]
 
 CNS_OK, CNS_EINVAL, CNS_EHIP, CNS_ENOMEM, CNS_ENUMERIC, CNS_EBUDGET, CNS_EIO = 0, -1, -2, -3, -4, -5, -6
 CNS_IMPL_TILED, CNS_IMPL_DIRECT = 0, 1
@@ -96,6 +96,13 @@ _SIGS = {
     "cns_batch_get_tile": (_INT, [_VP, _INT, C.POINTER(_INT), C.POINTER(_INT)]),
     "cns_batch_stats": (_INT, [_VP, C.POINTER(C.c_size_t), C.POINTER(_INT), C.POINTER(_INT)]),
     "cns_batch_set_target": (_INT, [_VP, _INT, C.c_longlong]),
+    "cns_domain_create": (_INT, [C.POINTER(_VP), _I3, _INT, _D3, _DBL, _DBL, _DBL]),
+    "cns_domain_destroy": (_INT, [_VP]),
+    "cns_domain_boxes": (_INT, [_VP, _I3, C.POINTER(_INT)]),
+    "cns_domain_box": (_INT, [_VP, _INT, _I3, _I3]),
+    "cns_domain_array_len": (C.c_size_t, [_VP, _INT, _INT]),
+    "cns_domain_fill_boundary": (_INT, [_VP, _VP]),
+    "cns_domain_advance": (_INT, [_VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(C.c_double)]),
     "cns_read_general_input": (_INT, [C.c_char_p, C.POINTER(Config)]),
     "cns_write_general_input": (_INT, [C.c_char_p, C.POINTER(Config)]),
     "cns_read_field": (_INT, [C.c_char_p, _I3, _INT, _INT, _VP]),

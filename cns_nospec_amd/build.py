@@ -25,7 +25,7 @@ DRIVER = os.path.join(PKG, "cns_run")
 ARCH = "gfx950"
 
 SOURCES = ["cns_kernels.hip", "cns_stencil.hip", "cns_abi.hip", "cns_batch.hip",
-           "cns_batch_engine.hip", "cns_io.cpp", "cns_stencil_pipe.hip"]
+           "cns_batch_engine.hip", "cns_io.cpp", "cns_stencil_pipe.hip", "cns_domain.hip"]
 
 
 def hipcc() -> str:

@@ -410,10 +410,11 @@ __device__ __forceinline__ void diff_tile(BoxDev *__restrict__ boxes, int b, int
 // cpu/hypterm.cpp:11-209 on the stage state, then per cell the SSP-RK3 update in
 // FP64 with the tile's D (cpu/advance.cpp:82, :114-116, :148-150), ctoprim of the
 // result into the next stage's primitives, and in stage 3 the Courant maximum.
-// Returns the lane's Courant maximum (bits; stage 3 only).
-template <typename FT, int LTX>
+// Returns the lane's Courant maximum (bits; stage 3 only).  The stage is a template
+// parameter: stage 1 then has no base operand at all (its base is the stage state).
+template <typename FT, int LTX, int stage>
 __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxes, int b, int x0,
-                                                       int y0, int z0, int nz, int stage, FT *lds)
+                                                       int y0, int z0, int nz, FT *lds)
 {
     using KK = K<FT>;
     constexpr int TXr = 1 << LTX, TYr = kBatchThreads >> LTX, P = TXr + 8;
@@ -510,8 +511,10 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
             rho_c = c_r[zo];
 #pragma unroll
             for (int l = 0; l < 4; l++) rkd[l] = (double)din[l * e3 + zo];
+            if (stage != 1) {
 #pragma unroll
-            for (int l = 0; l < NC; l++) rkb[l] = b_0[l * e3 + zo];
+                for (int l = 0; l < NC; l++) rkb[l] = b_0[l * e3 + zo];
+            }
         }
         if (t.active) {
             FT F[5];
@@ -555,15 +558,16 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
             F[IMZ] = F[IMZ] - (KHMOMP(ZMZ, ZW, ZP)) * dxi2;
             F[IENE] = F[IENE] - (KHENE(ZE, ZW, ZPF)) * dxi2;
             FENCE();
-            // one branch-free form for the three stages: out = ra*base + rb*(cur + dt*(D+F))
-            // with (ra, rb) = (0, 1), (3/4, 1/4), (1/3, 2/3); for stage 1, 0*base + 1*x
-            // is exactly x, so every stage keeps the reference's rounding.
+            // stage 1: out = cur + dt*(D+F) (cpu/advance.cpp:82); stages 2 and 3:
+            // out = ra*base + rb*(cur + dt*(D+F)), (ra, rb) = (3/4, 1/4) and (1/3, 2/3)
+            // (:114-116, :148-150)
             double o[5];
 #pragma unroll
             for (int l = 0; l < NC; l++) {
                 const double dl = l == 0 ? 0.0E0 : rkd[l - 1];
                 const double df = dl + (double)F[l];
-                o[l] = ra * rkb[l] + rb * ((l == 0 ? rho_c : cur_c[l]) + dt * df);
+                const double x = (l == 0 ? rho_c : cur_c[l]) + dt * df;
+                o[l] = stage == 1 ? x : ra * rkb[l] + rb * x;
                 o_0[l * e3 + zo] = o[l];
             }
             double u, v, w, p, T;
@@ -629,15 +633,17 @@ batch_diff_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles,
     }
 }
 
-template <typename FT, int LTX>
-__global__ void __launch_bounds__(kBatchThreads, 2)
-batch_hyp_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles, int stage)
+// The FP32 stage-1 tile (no base operand, no Courant reduction) fits 3 waves per SIMD
+// without spilling; the others 2.
+template <typename FT, int LTX, int stage>
+__global__ void __launch_bounds__(kBatchThreads, (sizeof(FT) == 4 && stage == 1) ? 3 : 2)
+batch_hyp_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles)
 {
     __shared__ FT lds[7 * kLdsCells];
     for (int ti = xcd_rank(blockIdx.x, gridDim.x); ti < ntiles; ti += gridDim.x) {
         const TileArgs a = read_tile(tiles, ti);
         if (box_stopped(boxes[a.b])) continue;  // uniform over the workgroup
-        const unsigned long long cm = hyp_tile<FT, LTX>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, stage, lds);
+        const unsigned long long cm = hyp_tile<FT, LTX, stage>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, lds);
         if (stage == 3) {
             const unsigned long long m = block_max_u64(cm);
             if (threadIdx.x == 0) atomicMax(&boxes[a.b].acc, m);
@@ -652,7 +658,11 @@ hipError_t stage_class(BoxDev *boxes, const TileDev *tiles, int n, int grid, int
     if (diff)
         batch_diff_kernel<FT, LTX><<<grid, kBatchThreads, 0, s>>>(boxes, tiles, n, stage);
     else
-        batch_hyp_kernel<FT, LTX><<<grid, kBatchThreads, 0, s>>>(boxes, tiles, n, stage);
+        switch (stage) {
+        case 1: batch_hyp_kernel<FT, LTX, 1><<<grid, kBatchThreads, 0, s>>>(boxes, tiles, n); break;
+        case 2: batch_hyp_kernel<FT, LTX, 2><<<grid, kBatchThreads, 0, s>>>(boxes, tiles, n); break;
+        default: batch_hyp_kernel<FT, LTX, 3><<<grid, kBatchThreads, 0, s>>>(boxes, tiles, n); break;
+        }
     return hipGetLastError();
 }
 
@@ -668,11 +678,15 @@ int occupancy(KFN kfn)
     return per < 1 ? 1 : per;
 }
 
+// Persistent grids of class c: grid[c] for its diffterm kernel, grid[3 * s + c]
+// for its stage-s hypterm kernel (s = 1, 2, 3).
 template <typename FT, int LTX>
-void class_grids(int cus, int &gd, int &gh)
+void class_grids(int cus, int c, int grid[12])
 {
-    gd = cus * occupancy(batch_diff_kernel<FT, LTX>);
-    gh = cus * occupancy(batch_hyp_kernel<FT, LTX>);
+    grid[c] = cus * occupancy(batch_diff_kernel<FT, LTX>);
+    grid[3 + c] = cus * occupancy(batch_hyp_kernel<FT, LTX, 1>);
+    grid[6 + c] = cus * occupancy(batch_hyp_kernel<FT, LTX, 2>);
+    grid[9 + c] = cus * occupancy(batch_hyp_kernel<FT, LTX, 3>);
 }
 
 }  // namespace
@@ -697,14 +711,14 @@ hipError_t launch_batch_finalize(BoxDev *boxes, const int *ids, int n, int count
 }
 
 hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int start[3],
-                              const int count[3], const int grid[6], int stage, bool fp32, hipStream_t s)
+                              const int count[3], const int grid[12], int stage, bool fp32, hipStream_t s)
 {
     for (int pass = 0; pass < 2; pass++) {  // all diffterm launches, then all hypterm launches
         const bool diff = pass == 0;
         for (int c = 0; c < 3; c++) {
             const int n = count[c];
             if (n <= 0) continue;
-            const int gc = grid[pass * 3 + c];
+            const int gc = grid[pass == 0 ? c : 3 * stage + c];
             const int g = gc < n ? gc : n;
             const TileDev *t = tiles + start[c];
             hipError_t e;
@@ -722,7 +736,7 @@ hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int sta
     return hipSuccess;
 }
 
-void batch_class_grids(bool fp32, int grid[6])
+void batch_class_grids(bool fp32, int grid[12])
 {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
@@ -730,13 +744,13 @@ void batch_class_grids(bool fp32, int grid[6])
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
     }
     if (fp32) {
-        class_grids<float, 4>(cus, grid[0], grid[3]);
-        class_grids<float, 5>(cus, grid[1], grid[4]);
-        class_grids<float, 6>(cus, grid[2], grid[5]);
+        class_grids<float, 4>(cus, 0, grid);
+        class_grids<float, 5>(cus, 1, grid);
+        class_grids<float, 6>(cus, 2, grid);
     } else {
-        class_grids<double, 4>(cus, grid[0], grid[3]);
-        class_grids<double, 5>(cus, grid[1], grid[4]);
-        class_grids<double, 6>(cus, grid[2], grid[5]);
+        class_grids<double, 4>(cus, 0, grid);
+        class_grids<double, 5>(cus, 1, grid);
+        class_grids<double, 6>(cus, 2, grid);
     }
 }
 

@@ -119,7 +119,7 @@ struct cns_batch {
     std::vector<int> hids;
     std::map<int, std::pair<int, int>> tune;  // edge -> (width, kz)
     int cls_start[kTileClasses] = {0, 0, 0}, cls_count[kTileClasses] = {0, 0, 0};
-    int grid[2 * kTileClasses] = {0, 0, 0, 0, 0, 0};  // diffterm, then hypterm launches
+    int grid[4 * kTileClasses] = {};  // diffterm, then the hypterm launches of stages 1..3
     int nresident = 0, ntiles = 0;
     bool dirty = true;
     hipStream_t cap_stream = nullptr;

@@ -5,11 +5,37 @@
 
 namespace cns {
 
+// A set of n boxes of one geometry stored back to back: box b of an array starts
+// b * stride doubles after box 0, with stride su for 5-component ghosted arrays (U),
+// sq for 6-component ghosted ones (Q), sc for 5-component interior ones (D, F) and
+// sw for the one-component ghosted wz field.  The default is a single box.
+struct BoxSet {
+    int n = 1;
+    int64_t su = 0, sq = 0, sc = 0, sw = 0;
+};
+inline BoxSet box_set(const Geom &g, int n)
+{
+    BoxSet b;
+    b.n = n;
+    b.su = NC * g.cg;
+    b.sq = NQ * g.cg;
+    b.sc = NC * g.c;
+    b.sw = g.cg;
+    return b;
+}
+
 // Ghost shell, closed-form periodic image (cns_kernels.hip).
 hipError_t launch_fill_boundary(const Geom &g, double *u, hipStream_t s);
 // The same periodic fill of u's NC components and of the components of q selected by
 // qmask (bit l for component l), in one pass.
 hipError_t launch_fill_boundary_q(const Geom &g, double *u, double *q, unsigned qmask, hipStream_t s);
+// The periodic fill of a domain of nb[0] x nb[1] x nb[2] boxes (box b = I + nb[0]*(J +
+// nb[1]*K) at cell origin (I*n0, J*n1, K*n2)): each ghost cell of each box gets the
+// domain cell at the periodic image of its global index, from whichever box holds
+// it (multifab_fill_boundary, fortran90/advance.f90:81); u and the qmask components
+// of q, as launch_fill_boundary_q.  With one box it equals launch_fill_boundary_q.
+hipError_t launch_domain_fill(const Geom &g, const BoxSet &bs, const int nb[3], double *u, double *q,
+                              unsigned qmask, hipStream_t s);
 
 // Resets a Courant accumulator slot to the bits of max(init, 0) (cns_kernels.hip).
 hipError_t launch_courno_reset(unsigned long long *acc, double init, hipStream_t s);
@@ -54,6 +80,8 @@ hipError_t launch_diffterm_tiled(const Geom &g, const double *q, double *difflux
 // later).  diffterm reads wz, a one-component ghosted array filled by launch_wz from
 // the same q.
 bool pipe_aligned(const void *p);
+
+
 hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *q, double *flux,
                                double dxi0, double dxi1, double dxi2, const double *d,
                                const double *base, double *out, const double *dt_dev, int phase,
@@ -65,17 +93,18 @@ hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *
 hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, double dxi0,
                                double dxi1, double dxi2, const double *d, const double *base,
                                double *out, const double *dt_dev, int phase, bool write_f,
-                               double *qout, int qout_kind, hipStream_t s);
-hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hipStream_t s);
+                               double *qout, int qout_kind, hipStream_t s, const BoxSet &bs = BoxSet());
+hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hipStream_t s,
+                     const BoxSet &bs = BoxSet());
 // ctoprim (as launch_ctoprim, full or u, v, w, T only) that also writes the wz field of
 // launch_wz from the same state, in one z-marching pass.
 hipError_t launch_ctoprim_wz(const Geom &g, const double *u, double *q, double *wz,
                              unsigned long long *acc, double dx0, double dx1, double dx2, double dxi2,
-                             bool full, hipStream_t s);
+                             bool full, hipStream_t s, const BoxSet &bs = BoxSet());
 // write_rho = false leaves difflux's rho row (always 0.0) unwritten, for a caller that
 // stores it once and never reads it in between.
 hipError_t launch_diffterm_pipe(const Geom &g, const double *q, const double *wz, double *difflux,
                                 double dxi0, double dxi1, double dxi2, double eta, double alam,
-                                hipStream_t s, bool write_rho = true);
+                                hipStream_t s, bool write_rho = true, const BoxSet &bs = BoxSet());
 
 }  // namespace cns

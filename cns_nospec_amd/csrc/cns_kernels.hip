@@ -77,6 +77,78 @@ fill_boundary_kernel(Geom g, double *__restrict__ u, double *__restrict__ q, uns
     }
 }
 
+// Ghost cell h of the compact shell enumeration of fill_boundary_kernel, as (i, j, k).
+__device__ __forceinline__ void shell_cell(const Geom &g, int64_t e, int64_t elems_a, int &i, int &j, int &k)
+{
+    const int64_t plane_rows = (int64_t)2 * NG * g.g1;
+    if (e < elems_a) {
+        int64_t r = e / g.g0;
+        i = (int)(e - r * g.g0);
+        if (r < plane_rows) {
+            const int kk = (int)(r / g.g1);
+            j = (int)(r - (int64_t)kk * g.g1);
+            k = kk < NG ? kk : g.n2 + kk;
+        } else {
+            r -= plane_rows;
+            const int kk = (int)(r / (2 * NG));
+            const int jj = (int)(r - (int64_t)kk * 2 * NG);
+            k = NG + kk;
+            j = jj < NG ? jj : g.n1 + jj;
+        }
+    } else {
+        const int64_t e2 = e - elems_a;
+        const int64_t row = e2 / (2 * NG);
+        const int ii = (int)(e2 - row * 2 * NG);
+        i = ii < NG ? ii : g.n0 + ii;
+        const int kk = (int)(row / g.n1);
+        k = NG + kk;
+        j = NG + (int)(row - (int64_t)kk * g.n1);
+    }
+}
+
+// One axis of the domain fill: ghost index a (0 <= a < n + 2 NG) of box I of nb
+// boxes of n cells -> the box I' and ghosted index a' holding the periodic image.
+__device__ __forceinline__ void domain_image(int a, int I, int n, int nb, int &Is, int &as)
+{
+    const int N = n * nb;
+    int G = I * n + a - NG;  // global cell index, in [-NG, N + NG)
+    G = G < 0 ? G + N : (G >= N ? G - N : G);
+    Is = G / n;
+    as = G - Is * n + NG;
+}
+
+// Domain fill: one y-row of the grid per box (blockIdx.y), grid-stride over its shell.
+__global__ void __launch_bounds__(kStreamBlock)
+domain_fill_kernel(Geom g, BoxSet bs, int nbx, int nby, int nbz, double *__restrict__ u,
+                   double *__restrict__ q, unsigned qmask, int64_t n_rows_a, int64_t total)
+{
+    const int b = blockIdx.y;
+    const int I = b % nbx, J = (b / nbx) % nby, K = b / (nbx * nby);
+    const int64_t elems_a = n_rows_a * g.g0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        shell_cell(g, e, elems_a, i, j, k);
+        int Is, Js, Ks, is, js, ks;
+        domain_image(i, I, g.n0, nbx, Is, is);
+        domain_image(j, J, g.n1, nby, Js, js);
+        domain_image(k, K, g.n2, nbz, Ks, ks);
+        const int bsrc = Is + nbx * (Js + nby * Ks);
+        const int64_t dst = gidx(g, i, j, k), src = gidx(g, is, js, ks);
+        double *ud = u + b * bs.su;
+        const double *us = u + bsrc * bs.su;
+#pragma unroll
+        for (int l = 0; l < NC; l++) ud[l * g.cg + dst] = us[l * g.cg + src];
+        if (qmask) {
+            double *qd = q + b * bs.sq;
+            const double *qs = q + bsrc * bs.sq;
+#pragma unroll
+            for (int l = 0; l < NQ; l++)
+                if (qmask & (1u << l)) qd[l * g.cg + dst] = qs[l * g.cg + src];
+        }
+    }
+}
+
 // -------------------------------------------------------------------- courno
 __global__ void courno_reset_kernel(unsigned long long *acc, double init)
 {
@@ -356,6 +428,22 @@ hipError_t launch_fill_boundary_q(const Geom &g, double *u, double *q, unsigned 
     const int64_t rows_a = (int64_t)2 * NG * g.g1 + (int64_t)g.n2 * 2 * NG;
     const int64_t total = rows_a * g.g0 + (int64_t)g.n1 * g.n2 * 2 * NG;
     fill_boundary_kernel<<<stream_grid(total, 2), kStreamBlock, 0, s>>>(g, u, q, qmask & 63u, rows_a, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_domain_fill(const Geom &g, const BoxSet &bs, const int nb[3], double *u, double *q,
+                              unsigned qmask, hipStream_t s)
+{
+    if (!q) qmask = 0;
+    if (bs.n != nb[0] * nb[1] * nb[2] || bs.n < 1 || bs.n > 65535) return hipErrorInvalidValue;
+    const int64_t rows_a = (int64_t)2 * NG * g.g1 + (int64_t)g.n2 * 2 * NG;
+    const int64_t total = rows_a * g.g0 + (int64_t)g.n1 * g.n2 * 2 * NG;
+    int gx = stream_grid(total, 2);
+    // a few waves per box keep the whole grid resident on small boxes
+    const int64_t want = (int64_t)(gx + bs.n - 1) / bs.n;
+    gx = (int)(want < 1 ? 1 : want);
+    const dim3 grid((unsigned)gx, (unsigned)bs.n);
+    domain_fill_kernel<<<grid, kStreamBlock, 0, s>>>(g, bs, nb[0], nb[1], nb[2], u, q, qmask & 63u, rows_a, total);
     return hipGetLastError();
 }
 

@@ -350,12 +350,21 @@ constexpr int CROSS = XSTRIP + 8 * TX;      // + rows 0..3 and 12..15, columns 4
 template <bool STAGE1, bool WRITE_F, int QOUT>
 __global__ void __launch_bounds__(NT, 2)
 hypterm_cons_kernel(Geom g, const double *__restrict__ cons, double *__restrict__ flux, double dxi0,
-                    double dxi1, double dxi2, int kz, int ntx, int nty, PipeRk rk)
+                    double dxi1, double dxi2, int kz, int ntx, int nty, PipeRk rk, BoxSet bs, int nbpb)
 {
     __shared__ double sm[2 * CNF * TILE];
     __shared__ double sd[SDNF * TILE];
 
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    // box of a box set (nbpb workgroups each), then the tile within it
+    int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int box = t / nbpb;
+    t -= box * nbpb;
+    cons += box * bs.su;
+    flux += box * bs.sc;
+    rk.d += box * bs.sc;
+    rk.base += box * bs.su;
+    rk.out += box * bs.su;
+    if (rk.qout) rk.qout += box * bs.sq;
     const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const TileDma td{bx * TX, by * BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
@@ -468,8 +477,11 @@ template <bool COURNO, bool FULL>
 __global__ void __launch_bounds__(kCtoBlock)
 ctoprim_wz_kernel(Geom g, const double *__restrict__ u, double *__restrict__ q, double *__restrict__ wz,
                   unsigned long long *__restrict__ acc, double dx0, double dx1, double dx2, double dxi2,
-                  int kz)
+                  int kz, BoxSet bs)
 {
+    u += blockIdx.z * bs.su;  // box of a box set
+    q += blockIdx.z * bs.sq;
+    wz += blockIdx.z * bs.sw;
     const int64_t pcol = (int64_t)blockIdx.x * kCtoBlock + threadIdx.x;  // j*xg + i
     const int i = (int)(pcol % g.xg), j = (int)(pcol / g.xg);
     const bool col = pcol < g.pg && i < g.g0;
@@ -545,8 +557,10 @@ ctoprim_wz_kernel(Geom g, const double *__restrict__ u, double *__restrict__ q, 
 // interior k (cpu/diffterm.cpp:126-151), into a one-component array with the ghosted
 // layout (plane k+ng).  One thread per column and run of kz planes, 9-plane window.
 __global__ void __launch_bounds__(256)
-wz_kernel(Geom g, const double *__restrict__ q_w, double *__restrict__ wz, double dxi2, int kz)
+wz_kernel(Geom g, const double *__restrict__ q_w, double *__restrict__ wz, double dxi2, int kz, BoxSet bs)
 {
+    q_w += blockIdx.z * bs.sq;  // box of a box set
+    wz += blockIdx.z * bs.sw;
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;  // j*xg + i
     if (p >= g.pg || (int)(p % g.xg) >= g.g0) return;
     const int kbeg = blockIdx.y * kz, kend = min(kbeg + kz, g.n2);
@@ -578,13 +592,18 @@ template <bool WRITE_RHO>
 __global__ void __launch_bounds__(NT, 2)
 diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restrict__ wzf,
                      double *__restrict__ difflux, double dxi0, double dxi1, double dxi2, double eta,
-                     double alam, int kz, int ntx, int nty)
+                     double alam, int kz, int ntx, int nty, BoxSet bs, int nbpb)
 {
     __shared__ double sm[2 * DNF * TILE];
     __shared__ double s_ux[R * TX];   // row r (0..15), column 4+c
     __shared__ double s_vy[BY * PX];  // row 4+r (r 0..7), column c (0..71)
 
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int box = t / nbpb;
+    t -= box * nbpb;
+    q += box * bs.sq;
+    wzf += box * bs.sw;
+    difflux += box * bs.sc;
     const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const TileDma td{bx * TX, by * BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
@@ -796,10 +815,10 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
 // z runs: enough workgroups to fill 256 CUs several times over (one 512-thread
 // workgroup per CU fits the LDS), with runs long enough that the 8-plane window
 // warm-up stays a small overhead.
-int pipe_kz(const Geom &g, int tiles_xy)
+int pipe_kz(const Geom &g, int tiles_xy, int nbox = 1)
 {
     int kz = g.n2;
-    while (kz > 16 && (int64_t)tiles_xy * ((g.n2 + kz - 1) / kz) < 2048) kz = (kz + 1) / 2;
+    while (kz > 16 && (int64_t)tiles_xy * nbox * ((g.n2 + kz - 1) / kz) < 2048) kz = (kz + 1) / 2;
     return kz;
 }
 
@@ -832,17 +851,19 @@ hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *
 hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, double dxi0,
                                double dxi1, double dxi2, const double *d, const double *base,
                                double *out, const double *dt_dev, int phase, bool write_f,
-                               double *qout, int qout_kind, hipStream_t s)
+                               double *qout, int qout_kind, hipStream_t s, const BoxSet &bs)
 {
     const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
-    const int kz = pipe_kz(g, ntx * nty);
-    const int nb = ntx * nty * ((g.n2 + kz - 1) / kz);
+    const int kz = pipe_kz(g, ntx * nty, bs.n);
+    const int nbpb = ntx * nty * ((g.n2 + kz - 1) / kz);
+    if (bs.n < 1 || (int64_t)nbpb * bs.n > 0x7fffffff) return hipErrorInvalidValue;
+    const int nb = nbpb * bs.n;
     PipeRk rk{d, base, out, dt_dev, phase, qout};
     const bool stage1 = phase == 1 && base == cons;
     if (!qout) qout_kind = 0;
     if (qout_kind < 0 || qout_kind > 2 || (write_f && qout_kind)) return hipErrorInvalidValue;
 #define HYP_CONS(S1, WF, QO) \
-    hypterm_cons_kernel<S1, WF, QO><<<nb, NT, 0, s>>>(g, cons, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk)
+    hypterm_cons_kernel<S1, WF, QO><<<nb, NT, 0, s>>>(g, cons, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk, bs, nbpb)
     // the step's combinations: stage 1 (-> q 1), stage 2 (-> q 2), stage 3 (F, no q);
     // and no q for any stage
     if (stage1 && !write_f)
@@ -859,11 +880,14 @@ hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, 
 
 hipError_t launch_ctoprim_wz(const Geom &g, const double *u, double *q, double *wz,
                              unsigned long long *acc, double dx0, double dx1, double dx2, double dxi2,
-                             bool full, hipStream_t s)
+                             bool full, hipStream_t s, const BoxSet &bs)
 {
     const int kz = 64;
-    const dim3 grid((unsigned)((g.pg + kCtoBlock - 1) / kCtoBlock), (unsigned)((g.g2 + kz - 1) / kz));
-#define CTO_WZ(C, F) ctoprim_wz_kernel<C, F><<<grid, kCtoBlock, 0, s>>>(g, u, q, wz, acc, dx0, dx1, dx2, dxi2, kz)
+    if (bs.n < 1 || bs.n > 65535) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((g.pg + kCtoBlock - 1) / kCtoBlock), (unsigned)((g.g2 + kz - 1) / kz),
+                    (unsigned)bs.n);
+#define CTO_WZ(C, F) \
+    ctoprim_wz_kernel<C, F><<<grid, kCtoBlock, 0, s>>>(g, u, q, wz, acc, dx0, dx1, dx2, dxi2, kz, bs)
     if (acc)
         full ? CTO_WZ(true, true) : CTO_WZ(true, false);
     else
@@ -872,25 +896,28 @@ hipError_t launch_ctoprim_wz(const Geom &g, const double *u, double *q, double *
     return hipGetLastError();
 }
 
-hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hipStream_t s)
+hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hipStream_t s, const BoxSet &bs)
 {
     const int kz = g.n2 > 128 ? 128 : g.n2;
-    const dim3 grid((unsigned)((g.pg + 255) / 256), (unsigned)((g.n2 + kz - 1) / kz));
-    wz_kernel<<<grid, 256, 0, s>>>(g, q + QW * g.cg, wz, dxi2, kz);
+    if (bs.n < 1 || bs.n > 65535) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((g.pg + 255) / 256), (unsigned)((g.n2 + kz - 1) / kz), (unsigned)bs.n);
+    wz_kernel<<<grid, 256, 0, s>>>(g, q + QW * g.cg, wz, dxi2, kz, bs);
     return hipGetLastError();
 }
 
 hipError_t launch_diffterm_pipe(const Geom &g, const double *q, const double *wz, double *difflux,
                                 double dxi0, double dxi1, double dxi2, double eta, double alam,
-                                hipStream_t s, bool write_rho)
+                                hipStream_t s, bool write_rho, const BoxSet &bs)
 {
     const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
-    const int kz = pipe_kz(g, ntx * nty);
-    const int nb = ntx * nty * ((g.n2 + kz - 1) / kz);
+    const int kz = pipe_kz(g, ntx * nty, bs.n);
+    const int nbpb = ntx * nty * ((g.n2 + kz - 1) / kz);
+    if (bs.n < 1 || (int64_t)nbpb * bs.n > 0x7fffffff) return hipErrorInvalidValue;
+    const int nb = nbpb * bs.n;
     if (write_rho)
-        diffterm_pipe_kernel<true><<<nb, NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty);
+        diffterm_pipe_kernel<true><<<nb, NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty, bs, nbpb);
     else
-        diffterm_pipe_kernel<false><<<nb, NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty);
+        diffterm_pipe_kernel<false><<<nb, NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty, bs, nbpb);
     return hipGetLastError();
 }
 
