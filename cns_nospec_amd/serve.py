@@ -67,25 +67,33 @@ def make_stream(n: int, rate: float, seed: int, edge_range=(24, 128), steps_rang
 
 
 def request_state(req: Request, device="cuda"):
-    """The request's initial ghost-free U (5, e, e, e) on the device."""
+    """The request's initial ghost-free U (5, e, e, e) on the device.
+
+    The five primitive fields are evaluated together (one set of launches for all
+    of them, no host synchronisation), mode by mode in the order the modes are drawn."""
     from .state import GAMMA, _draw_modes
     import torch
     e = req.edge
     rng = np.random.default_rng(req.seed)
     crd = (torch.arange(e, dtype=torch.float64, device=device) + 0.5) / e
     c0 = math.sqrt(GAMMA)
-    prim = {}
-    for name, f0, amp in (("rho", 1.0, 0.05), ("u", 0.0, req.mach * c0), ("v", 0.0, req.mach * c0),
-                          ("w", 0.0, req.mach * c0), ("p", 1.0, 0.05)):
-        s = torch.zeros((e, e, e), dtype=torch.float64, device=device)
-        for k, ph, a in _draw_modes(rng, 2, 4):
-            fx = torch.sin(2 * math.pi * int(k[0]) * crd + float(ph[0]))
-            fy = torch.sin(2 * math.pi * int(k[1]) * crd + float(ph[1]))
-            fz = torch.sin(2 * math.pi * int(k[2]) * crd + float(ph[2]))
-            s.add_(float(a) * fz[:, None, None] * fy[None, :, None] * fx[None, None, :])
-        s.mul_(amp / float(s.abs().max())).add_(f0)
-        prim[name] = s
-    rho, u, v, w, p = prim["rho"], prim["u"], prim["v"], prim["w"], prim["p"]
+    f0 = torch.tensor([1.0, 0.0, 0.0, 0.0, 1.0], dtype=torch.float64, device=device)
+    amp = torch.tensor([0.05] + [req.mach * c0] * 3 + [0.05], dtype=torch.float64, device=device)
+    modes = [_draw_modes(rng, 2, 4) for _ in range(5)]  # rho, u, v, w, p
+    s = torch.zeros((5, e, e, e), dtype=torch.float64, device=device)
+    for m in range(4):
+        k = torch.tensor([[int(modes[f][m][0][d]) for d in range(3)] for f in range(5)],
+                         dtype=torch.float64, device=device)
+        ph = torch.tensor([[float(modes[f][m][1][d]) for d in range(3)] for f in range(5)],
+                          dtype=torch.float64, device=device)
+        a = torch.tensor([float(modes[f][m][2]) for f in range(5)], dtype=torch.float64, device=device)
+        fx = torch.sin(2 * math.pi * k[:, 0:1] * crd + ph[:, 0:1])
+        fy = torch.sin(2 * math.pi * k[:, 1:2] * crd + ph[:, 1:2])
+        fz = torch.sin(2 * math.pi * k[:, 2:3] * crd + ph[:, 2:3])
+        s.add_(a[:, None, None, None] * fz[:, :, None, None] * fy[:, None, :, None] * fx[:, None, None, :])
+    smax = s.abs().amax(dim=(1, 2, 3))
+    s.mul_((amp / smax)[:, None, None, None]).add_(f0[:, None, None, None])
+    rho, u, v, w, p = s[0], s[1], s[2], s[3], s[4]
     return torch.stack([rho, rho * u, rho * v, rho * w,
                         p / (GAMMA - 1.0) + 0.5 * rho * (u * u + v * v + w * w)]).contiguous()
 
