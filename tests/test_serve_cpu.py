@@ -1,5 +1,6 @@
-"""Host logic of the serving layer (no GPU): the seeded request stream and the
-latency/throughput summary."""
+"""Host logic of the serving layer (no GPU): the seeded request stream, the
+latency/throughput summary, and the batch server's admission loop over a host
+stand-in engine."""
 import math
 
 from cns_nospec_amd.serve import Result, make_stream, summarize
@@ -24,3 +25,70 @@ def test_summary_counts_only_successful_work():
     assert s["failed"] == 1 and s["cell_steps"] == 10 ** 3 * 5 + 20 ** 3 * 2
     assert math.isclose(s["cell_steps_per_s"], s["cell_steps"] / 4.0)
     assert s["p50_latency_s"] == 2.5
+
+
+class _FakeEngine:
+    """Host stand-in for BatchEngine: boxes advance by whole steps up to their target;
+    the pool holds `budget` bytes at edge^3 bytes per box."""
+    log = []
+
+    def __init__(self, hbm_budget, fp32=True, max_boxes=4096, tuning=None):
+        self.budget, self.used, self.boxes, self.next = hbm_budget, 0, {}, 0
+        self.resident_peak = 0
+        _FakeEngine.log = []
+
+    def add(self, u0, edge):
+        from cns_nospec_amd import _lib
+        if self.used + edge ** 3 > self.budget:
+            raise _lib.CnsBudgetError("no room (fake)", _lib.CNS_EBUDGET)
+        self.used += edge ** 3
+        bid = self.next
+        self.next += 1
+        self.boxes[bid] = {"edge": edge, "steps": 0, "target": 0}
+        _FakeEngine.log.append(("add", bid, edge))
+        self.resident_peak = max(self.resident_peak, sum(b["edge"] ** 3 for b in self.boxes.values()))
+        return bid
+
+    def set_target(self, bid, steps):
+        self.boxes[bid]["target"] = steps
+
+    def step(self, k):
+        for b in self.boxes.values():
+            b["steps"] = min(b["target"], b["steps"] + k) if b["target"] else b["steps"] + k
+
+    def query(self, ids):
+        return {i: {"steps": self.boxes[i]["steps"], "dt": 1.0, "courno": 1.0, "err": 0} for i in ids}
+
+    def read(self, bid):
+        raise AssertionError("no state kept in this test")
+
+    def remove(self, bid):
+        self.used -= self.boxes.pop(bid)["edge"] ** 3
+
+    def close(self):
+        pass
+
+
+class _Flat:
+    def reshape(self, *a):
+        return self
+
+
+def test_batch_server_admission_targets_and_cap(monkeypatch):
+    """FIFO admission under the HBM budget and the resident-cell cap, each box run to
+    exactly its own step target, every request served once."""
+    from cns_nospec_amd import batch, serve
+    monkeypatch.setattr(batch, "BatchEngine", _FakeEngine)
+    monkeypatch.setattr(serve, "request_state", lambda req: _Flat())
+    reqs = make_stream(40, 0.0, 11)
+    cap = 3 * 100 ** 3
+    srv = serve.BatchServer(10 * 128 ** 3, fp32=True, max_chunk=7, max_resident_cells=cap)
+    eng = srv.engine
+    out, summ = srv.run(reqs)
+    assert summ["requests"] == 40 and summ["failed"] == 0
+    assert [r.rid for r in out] == list(range(40))
+    assert [e[1] for e in _FakeEngine.log] == list(range(40))  # one admission each
+    assert [e[2] for e in _FakeEngine.log] == [r.edge for r in reqs]  # in arrival (FIFO) order
+    assert cap // 2 < eng.resident_peak <= cap  # the cap binds, and holds
+    assert all(r.done >= r.admitted >= r.arrival for r in out)
+    assert summ["cell_steps"] == sum(r.edge ** 3 * r.steps for r in reqs)
