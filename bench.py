@@ -160,6 +160,34 @@ def cpu_baseline(edge: int = 128, budget_s: float = 15.0):
                       f"{el:.1f} s; the path has no threading (its OpenMP is commented out)"}
 
 
+def cpu_baseline_all_cores(edge: int = 128, budget_s: float = 10.0):
+    """The oracle's CPU restatement of new_advance (same operation order, gcc -O2) on
+    every host core the process may use (at most 16, the host share of one GPU), with
+    OpenMP over its per-cell loops: the CPU path at full host width, beside the
+    reference's single-threaded one."""
+    import oracle
+    from cns_nospec_amd.layout import NC, NQ, BoxLayout
+    from cns_nospec_amd.state import ALAM, CFL, ETA, box_dx, sinusoid_state
+
+    orc = oracle.Oracle()
+    cores = min(16, len(os.sched_getaffinity(0)))
+    threads = orc.set_threads(cores)
+    lay = BoxLayout.cube(edge)
+    U = sinusoid_state(lay, 7, kmax=4)
+    Unew, Q = lay.new_ghosted(NC), lay.new_ghosted(NQ)
+    D, F = lay.new_interior(NC), lay.new_interior(NC)
+    dx = box_dx(edge)
+    n = 0
+    t0 = time.perf_counter()
+    while n < 2 or (time.perf_counter() - t0) < budget_s:
+        orc.advance(lay, dx, CFL, ETA, ALAM, U, Unew, Q, D, F)
+        n += 1
+    el = time.perf_counter() - t0
+    return {"value": edge ** 3 * n / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} RK3 steps of one {edge}^3 box (seeded sinusoids), {threads} OpenMP "
+                      f"threads, {el:.1f} s"}
+
+
 BATCH_EDGES = (16, 24, 32, 48, 64, 96)
 
 
@@ -491,6 +519,7 @@ def main():
         out["secondary"] = sec
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline_all_cores"] = cpu_baseline_all_cores()
     print(json.dumps(out), flush=True)
 
 

@@ -48,9 +48,14 @@ class Oracle:
             build(ref=False)
         lib = C.CDLL(path)
         for name in ("orc_fill_boundary", "orc_ctoprim", "orc_hypterm", "orc_diffterm",
-                     "orc_rk_update", "orc_advance"):
+                     "orc_rk_update", "orc_advance", "orc_set_threads"):
             getattr(lib, name).restype = C.c_int
         self.lib = lib
+
+    def set_threads(self, n: int = 0) -> int:
+        """OpenMP threads of the per-cell loops (n > 0 sets it); returns the count in use.
+        The results are the same for any count."""
+        return int(self.lib.orc_set_threads(int(n)))
 
     def fill_boundary(self, layout, u):
         dim = _I3(*layout.n)

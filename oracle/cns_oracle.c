@@ -12,10 +12,18 @@
  * the Courant maxima start at 0 instead of an uninitialised read
  * (cpu/ctoprim.cpp:21,57-59); every Courant number is positive, so the result
  * is the same closed-form maximum.
+ *
+ * The per-cell loops run on OpenMP threads (orc_set_threads); every cell is
+ * computed by the same expression from the same inputs whatever the thread, so
+ * the results do not depend on the thread count.  The Courant maximum stays a
+ * serial loop (its order decides which of several NaNs survives).
  */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "cns_oracle.h"
 
 /* cpu/header.h:14-45 */
@@ -75,6 +83,7 @@ int orc_fill_boundary(double *u, const int dim[3], const int dim_g[3])
     if (make_layout(lo, hi, NG, &L)) return -1;
     for (l = 0; l < NC; l++) {
         double *U = u + (long)l * L.cg;
+#pragma omp parallel for collapse(2) private(i)
         for (k = 0; k < NG; k++)
             for (j = NG; j < dim[1] + NG; j++)
                 for (i = NG; i < dim[0] + NG; i++) {
@@ -84,6 +93,7 @@ int orc_fill_boundary(double *u, const int dim[3], const int dim_g[3])
     }
     for (l = 0; l < NC; l++) {
         double *U = u + (long)l * L.cg;
+#pragma omp parallel for collapse(2) private(i)
         for (k = 0; k < dim_g[2]; k++)
             for (j = 0; j < NG; j++)
                 for (i = NG; i < dim[0] + NG; i++) {
@@ -93,6 +103,7 @@ int orc_fill_boundary(double *u, const int dim[3], const int dim_g[3])
     }
     for (l = 0; l < NC; l++) {
         double *U = u + (long)l * L.cg;
+#pragma omp parallel for collapse(2) private(i)
         for (k = 0; k < dim_g[2]; k++)
             for (j = 0; j < dim_g[1]; j++)
                 for (i = 0; i < NG; i++) {
@@ -116,6 +127,7 @@ int orc_ctoprim(const int lo[3], const int hi[3], const double *u, double *q,
     if (make_layout(lo, hi, ng, &L)) return -1;
     cg = L.cg;
 
+#pragma omp parallel for collapse(2) private(i)
     for (k = 0; k < L.dimg[2]; k++)
         for (j = 0; j < L.dimg[1]; j++)
             for (i = 0; i < L.dimg[0]; i++) {
@@ -202,6 +214,7 @@ int orc_hypterm(const int lo[3], const int hi[3], int ng, const double dx[3],
     if (make_layout(lo, hi, ng, &L)) return -1;
     for (d = 0; d < 3; d++) dxinv[d] = 1.0E0 / dx[d];
 
+#pragma omp parallel for collapse(2) private(i, l)
     for (k = lo[2]; k <= hi[2]; k++)
         for (j = lo[1]; j <= hi[1]; j++)
             for (i = lo[0]; i <= hi[0]; i++) {
@@ -276,11 +289,13 @@ int orc_diffterm(const int lo[3], const int hi[3], int ng, const double dx[3],
     uz = wy + L.cg; vz = uz + L.cg; wz = vz + L.cg;
 
     /* cpu/diffterm.cpp:63-68 */
+#pragma omp parallel for collapse(2) private(i)
     for (k = 0; k < L.n[2]; k++)
         for (j = 0; j < L.n[1]; j++)
             for (i = 0; i < L.n[0]; i++) difflux[IIDX(&L, k, j, i)] = 0.0E0;
 
     /* :73-97  x-derivatives on i interior, j and k ghosted */
+#pragma omp parallel for collapse(2) private(i)
     for (k = lo[2] - ng; k <= hi[2] + ng; k++)
         for (j = lo[1] - ng; j <= hi[1] + ng; j++)
             for (i = lo[0]; i <= hi[0]; i++) {
@@ -290,6 +305,7 @@ int orc_diffterm(const int lo[3], const int hi[3], int ng, const double dx[3],
                 wx[g] = d1(qW, g, sx) * dxinv[0];
             }
     /* :100-124  y-derivatives on j interior */
+#pragma omp parallel for collapse(2) private(i)
     for (k = lo[2] - ng; k <= hi[2] + ng; k++)
         for (j = lo[1]; j <= hi[1]; j++)
             for (i = lo[0] - ng; i <= hi[0] + ng; i++) {
@@ -299,6 +315,7 @@ int orc_diffterm(const int lo[3], const int hi[3], int ng, const double dx[3],
                 wy[g] = d1(qW, g, sy) * dxinv[1];
             }
     /* :127-151  z-derivatives on k interior */
+#pragma omp parallel for collapse(2) private(i)
     for (k = lo[2]; k <= hi[2]; k++)
         for (j = lo[1] - ng; j <= hi[1] + ng; j++)
             for (i = lo[0] - ng; i <= hi[0] + ng; i++) {
@@ -308,6 +325,7 @@ int orc_diffterm(const int lo[3], const int hi[3], int ng, const double dx[3],
                 wz[g] = d1(qW, g, sz) * dxinv[2];
             }
 
+#pragma omp parallel for collapse(2) private(i)
     for (k = lo[2]; k <= hi[2]; k++)
         for (j = lo[1]; j <= hi[1]; j++)
             for (i = lo[0]; i <= hi[0]; i++) {
@@ -376,6 +394,7 @@ int orc_rk_update(int phase, const int lo[3], const int hi[3], int ng,
     int i, j, k, l;
     if (make_layout(lo, hi, ng, &L)) return -1;
     if (phase < 1 || phase > 3) return -1;
+#pragma omp parallel for collapse(3) private(i)
     for (l = 0; l < NC; l++)
         for (k = 0; k < L.n[2]; k++)
             for (j = 0; j < L.n[1]; j++)
@@ -425,4 +444,16 @@ int orc_advance(int n, const double dx[3], double cfl, double eta, double alam,
     rc |= orc_hypterm(lo, hi, NG, dx, Unew, Q, F);
     rc |= orc_rk_update(3, lo, hi, NG, U, Unew, D, F, *dt);
     return rc ? -1 : 0;
+}
+
+/* ------------------------------------------------------------- threading */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }

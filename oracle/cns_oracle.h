@@ -45,6 +45,10 @@ int orc_rk_update(int phase, const int lo[3], const int hi[3], int ng,
 int orc_advance(int n, const double dx[3], double cfl, double eta, double alam,
                 double *U, double *Unew, double *Q, double *D, double *F, double *dt);
 
+/* Threads of the per-cell loops (n > 0 sets it); returns the count in use.  Results
+ * do not depend on it. */
+int orc_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif
