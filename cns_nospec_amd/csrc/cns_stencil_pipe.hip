@@ -1,8 +1,9 @@
 // cns_stencil_pipe.hip -- pipelined z-marching hypterm and diffterm for gfx950.
 //
-// A 512-thread workgroup owns a 64 x 8 column tile (one wave per row) and marches
-// up a run of planes.  Every field whose x/y neighbours a plane needs is staged in
-// LDS as a full 16 x 72 tile (8 rows / 4 columns of halo on each side), loaded with
+// A workgroup owns a TX x 8 column tile (TX = 64: 512 threads, one wave per row; the
+// step's fused kernels use TX = 32: 256 threads, two rows per wave, two workgroups per
+// CU) and marches up a run of planes.  Every field whose x/y neighbours a plane needs
+// is staged in LDS as a full 16 x (TX+8) tile (4 cells of halo on each side), loaded with
 // direct-to-LDS DMA (global_load_lds_dwordx4: each lane moves 16 bytes straight
 // into the tile, no registers involved) into one of two LDS buffers.  The loads of
 // plane k+1 are issued right after the barrier that opens plane k, so the whole
@@ -18,6 +19,8 @@
 // Arithmetic and its order are the reference's (cpu/hypterm.cpp:11-209,
 // cpu/diffterm.cpp:22-320, cpu/advance.cpp:78-153); with -ffp-contract=off the
 // results are bit-identical to the reference cpu/ path and to the other kernels.
+#include <cstdlib>
+
 #include "cns_kernels.h"
 
 namespace cns {
@@ -104,6 +107,42 @@ __device__ __forceinline__ void plane_barrier()
     __syncthreads();
 }
 
+// Tile shapes of the pipelined stencils: TXs x 8 cells per workgroup of TXs*8
+// threads.  At 64 x 8 diffterm takes 142 KB of LDS (one workgroup per CU); at 32 x 8
+// 77 KB, so two workgroups share a CU and one's barrier waits are the other's compute
+// time (likewise the fused hypterm: 117 KB and 65 KB).
+template <int TXs>
+struct Shape {
+    static constexpr int TX = TXs, BY = 8, PX = TX + 8, R = BY + 8, NT = TX * BY, NW = NT / 64;
+    static constexpr int TILE = R * PX, NDMA = TILE / 128;
+    static_assert(TILE % 128 == 0, "a tile is a whole number of 1 KB DMA blocks");
+    static_assert(PX % 2 == 0, "a lane's 16-byte pair stays in one row");
+};
+
+template <class S>
+__device__ __forceinline__ void dma_block_s(const double *src, double *tile, int n, const TileDma &t, int lane)
+{
+    const int e = 128 * n + 2 * lane;
+    const int r = e / S::PX, c = e - r * S::PX;
+    const int hj = min(t.gj0 + r, t.g1m1), hi = min(t.gi0 + c, t.xgm2);
+    const unsigned off = (unsigned)hj * t.xg + (unsigned)hi;
+    __builtin_amdgcn_global_load_lds((const void *)(src + off), (lds_void *)(tile + 128 * n), 16, 0, 0);
+}
+
+// This wave's share of the NF x NDMA DMA blocks of one plane: block n of field f goes
+// to wave (n + f) mod NW (NW waves; the last NDMA - NW blocks to the waves after them).
+template <class S, int NF>
+__device__ __forceinline__ void dma_plane_s(const double *const (&src)[NF], int64_t plane, double *buf,
+                                            const TileDma &t, int wave, int lane)
+{
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+        const int n0 = (wave - f) & (S::NW - 1);
+        dma_block_s<S>(src[f] + plane, buf + f * S::TILE, n0, t, lane);
+        if (n0 + S::NW < S::NDMA) dma_block_s<S>(src[f] + plane, buf + f * S::TILE, n0 + S::NW, t, lane);
+    }
+}
+
 struct PipeRk {
     const double *d;     // difflux (interior)
     const double *base;  // U^n (ghosted), phases 2 and 3
@@ -116,6 +155,7 @@ struct PipeRk {
 // hypterm's flux of the lane's cell (LDS index me of 16 x 72 tiles): x terms from the
 // row, y terms from the column, z terms from the register windows (plane k+m at
 // [4+m]), in the reference's order (cpu/hypterm.cpp:33-90, :101-147, :158-204).
+template <int P = PX>
 __device__ __forceinline__ void hyp_flux(const double *s_mx, const double *s_my, const double *s_mz,
                                          const double *s_E, const double *s_p, const double *s_u,
                                          const double *s_v, int me, const double (&wmx)[9],
@@ -137,12 +177,12 @@ __device__ __forceinline__ void hyp_flux(const double *s_mx, const double *s_my,
     F[IMZ] = -(HMOM(XMZ, XU)) * dxi0;
     F[IENE] = -(HENE(XE, XU, XP)) * dxi0;
     // ---- y: F = F - (sum)*dxinv(2)
-#define YMX(m) s_mx[me + (m) * PX]
-#define YMY(m) s_my[me + (m) * PX]
-#define YMZ(m) s_mz[me + (m) * PX]
-#define YE(m) s_E[me + (m) * PX]
-#define YP(m) s_p[me + (m) * PX]
-#define YV(m) s_v[me + (m) * PX]
+#define YMX(m) s_mx[me + (m) * P]
+#define YMY(m) s_my[me + (m) * P]
+#define YMZ(m) s_mz[me + (m) * P]
+#define YE(m) s_E[me + (m) * P]
+#define YP(m) s_p[me + (m) * P]
+#define YV(m) s_v[me + (m) * P]
     F[IRHO] = F[IRHO] - (D1(YMY)) * dxi1;
     F[IMX] = F[IMX] - (HMOM(YMX, YV)) * dxi1;
     F[IMY] = F[IMY] - (HMOMP(YMY, YV, YP)) * dxi1;
@@ -344,16 +384,17 @@ hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__res
 // kernel, for the same bits.
 enum { CRHO = 0, CMX, CMY, CMZ, CE, CNF };
 enum { SDU = 0, SDV, SDP, SDNF };
-constexpr int XSTRIP = BY * PX;             // rows 4..11, all 72 columns
-constexpr int CROSS = XSTRIP + 8 * TX;      // + rows 0..3 and 12..15, columns 4..67
 
-template <bool STAGE1, bool WRITE_F, int QOUT>
-__global__ void __launch_bounds__(NT, 2)
+template <class S, bool STAGE1, bool WRITE_F, int QOUT>
+__global__ void __launch_bounds__(S::NT, 2)
 hypterm_cons_kernel(Geom g, const double *__restrict__ cons, double *__restrict__ flux, double dxi0,
                     double dxi1, double dxi2, int kz, int ntx, int nty, PipeRk rk, BoxSet bs, int nbpb)
 {
-    __shared__ double sm[2 * CNF * TILE];
-    __shared__ double sd[SDNF * TILE];
+    constexpr int TILE_ = S::TILE, PX_ = S::PX;
+    constexpr int XSTRIP = S::BY * PX_;           // rows 4..11, all PX columns
+    constexpr int CROSS = XSTRIP + 8 * S::TX;    // + rows 0..3 and 12..15, columns 4..TX+3
+    __shared__ double sm[2 * CNF * TILE_];
+    __shared__ double sd[SDNF * TILE_];
 
     // box of a box set (nbpb workgroups each), then the tile within it
     int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -366,9 +407,10 @@ hypterm_cons_kernel(Geom g, const double *__restrict__ cons, double *__restrict_
     rk.out += box * bs.su;
     if (rk.qout) rk.qout += box * bs.sq;
     const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const TileDma td{bx * TX, by * BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
-    const int gi = td.gi0 + lane + 4, gj = td.gj0 + wave + 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;       // DMA issue
+    const int col = threadIdx.x % S::TX, row = threadIdx.x / S::TX;  // the lane's cell
+    const TileDma td{bx * S::TX, by * S::BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
+    const int gi = td.gi0 + col + 4, gj = td.gj0 + row + 4;
     const bool inbox = gi < g.g0 && gj < g.g1;
     const bool active = gi < g.n0 + NG && gj < g.n1 + NG;
     const int kbeg = bz * kz, kend = min(kbeg + kz, g.n2);
@@ -377,12 +419,12 @@ hypterm_cons_kernel(Geom g, const double *__restrict__ cons, double *__restrict_
                  *c_mz = cons + IMZ * cg, *c_E = cons + IENE * cg;
     const double *const src[CNF] = {c_rho, c_mx, c_my, c_mz, c_E};
     const int64_t own = (int64_t)gj * g.xg + gi;
-    const int me = (wave + 4) * PX + lane + 4;
+    const int me = (row + 4) * PX_ + col + 4;
     const double dt = rk.dt[0];
     const double ra = rk.phase == 1 ? 0.0E0 : (rk.phase == 2 ? ThreeQuarters : OneThird);
     const double rb = rk.phase == 1 ? 1.0E0 : (rk.phase == 2 ? OneQuarter : TwoThirds);
 
-    dma_plane<CNF>(src, (int64_t)(kbeg + NG) * pg, sm, td, wave, lane);
+    dma_plane_s<S, CNF>(src, (int64_t)(kbeg + NG) * pg, sm, td, wave, lane);
 
     double wmx[9], wmy[9], wmz[9], wE[9], ww[9], wp[9];
     double nrho = 1.0, nmx = 0, nmy = 0, nmz = 0, nE = 0;
@@ -400,9 +442,10 @@ hypterm_cons_kernel(Geom g, const double *__restrict__ cons, double *__restrict_
 
     for (int k = kbeg; k < kend; k++) {
         const int cur = (k - kbeg) & 1;
-        const double *b = sm + cur * (CNF * TILE);
+        const double *b = sm + cur * (CNF * TILE_);
         plane_barrier();
-        if (k + 1 < kend) dma_plane<CNF>(src, (int64_t)(k + 1 + NG) * pg, sm + (cur ^ 1) * (CNF * TILE), td, wave, lane);
+        if (k + 1 < kend)
+            dma_plane_s<S, CNF>(src, (int64_t)(k + 1 + NG) * pg, sm + (cur ^ 1) * (CNF * TILE_), td, wave, lane);
 
         const int64_t zo = (int64_t)(k + NG) * pg + own;
         {
@@ -426,31 +469,31 @@ hypterm_cons_kernel(Geom g, const double *__restrict__ cons, double *__restrict_
         }
 
         // u, v, p of the cross cells of this plane's tile
-        for (int h = threadIdx.x; h < CROSS; h += NT) {
+        for (int h = threadIdx.x; h < CROSS; h += S::NT) {
             int r, c;
             if (h < XSTRIP) {
-                r = 4 + h / PX;
-                c = h - (r - 4) * PX;
+                r = 4 + h / PX_;
+                c = h - (r - 4) * PX_;
             } else {
-                const int h2 = h - XSTRIP, rr = h2 >> 6;
-                r = rr < 4 ? rr : rr + BY;
-                c = 4 + (h2 & 63);
+                const int h2 = h - XSTRIP, rr = h2 / S::TX;
+                r = rr < 4 ? rr : rr + S::BY;
+                c = 4 + h2 % S::TX;
             }
-            const int l = r * PX + c;
+            const int l = r * PX_ + c;
             double u_, v_, w_, p_;
-            prim_uvwp(b[CRHO * TILE + l], b[CMX * TILE + l], b[CMY * TILE + l], b[CMZ * TILE + l],
-                      b[CE * TILE + l], u_, v_, w_, p_);
-            sd[SDU * TILE + l] = u_;
-            sd[SDV * TILE + l] = v_;
-            sd[SDP * TILE + l] = p_;
+            prim_uvwp(b[CRHO * TILE_ + l], b[CMX * TILE_ + l], b[CMY * TILE_ + l], b[CMZ * TILE_ + l],
+                      b[CE * TILE_ + l], u_, v_, w_, p_);
+            sd[SDU * TILE_ + l] = u_;
+            sd[SDV * TILE_ + l] = v_;
+            sd[SDP * TILE_ + l] = p_;
         }
         __syncthreads();
 
         if (active) {
             double F[5];
-            hyp_flux(b + CMX * TILE, b + CMY * TILE, b + CMZ * TILE, b + CE * TILE, sd + SDP * TILE,
-                     sd + SDU * TILE, sd + SDV * TILE, me, wmx, wmy, wmz, wE, ww, wp, dxi0, dxi1, dxi2, F);
-            const double uc[5] = {b[CRHO * TILE + me], wmx[4], wmy[4], wmz[4], wE[4]};
+            hyp_flux<PX_>(b + CMX * TILE_, b + CMY * TILE_, b + CMZ * TILE_, b + CE * TILE_, sd + SDP * TILE_,
+                          sd + SDU * TILE_, sd + SDV * TILE_, me, wmx, wmy, wmz, wE, ww, wp, dxi0, dxi1, dxi2, F);
+            const double uc[5] = {b[CRHO * TILE_ + me], wmx[4], wmy[4], wmz[4], wE[4]};
             double o[5];
             rk_store<STAGE1, WRITE_F>(F, uc, rkd, rkb, ra, rb, dt, flux, fo, g.c, rk.out, zo, cg, o);
             q_store<QOUT>(o, rk.qout, zo, cg);
@@ -588,15 +631,15 @@ wz_kernel(Geom g, const double *__restrict__ q_w, double *__restrict__ wz, doubl
 // Register windows: u, v, w, T and the computed ux, vy.
 enum { DCU = 0, DCV, DCW, DCT, DWZ, DLU, DLV, DNF };
 
-template <bool WRITE_RHO>
-__global__ void __launch_bounds__(NT, 2)
+template <class S, bool WRITE_RHO>
+__global__ void __launch_bounds__(S::NT, 2)
 diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restrict__ wzf,
                      double *__restrict__ difflux, double dxi0, double dxi1, double dxi2, double eta,
                      double alam, int kz, int ntx, int nty, BoxSet bs, int nbpb)
 {
-    __shared__ double sm[2 * DNF * TILE];
-    __shared__ double s_ux[R * TX];   // row r (0..15), column 4+c
-    __shared__ double s_vy[BY * PX];  // row 4+r (r 0..7), column c (0..71)
+    __shared__ double sm[2 * DNF * S::TILE];
+    __shared__ double s_ux[S::R * S::TX];   // row r (0..15), column 4+c
+    __shared__ double s_vy[S::BY * S::PX];  // row 4+r (r 0..7), column c (0..PX-1)
 
     int t = xcd_tile(blockIdx.x, gridDim.x);
     const int box = t / nbpb;
@@ -605,9 +648,10 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
     wzf += box * bs.sw;
     difflux += box * bs.sc;
     const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const TileDma td{bx * TX, by * BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
-    const int gi = td.gi0 + lane + 4, gj = td.gj0 + wave + 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;       // DMA issue
+    const int col = threadIdx.x % S::TX, row = threadIdx.x / S::TX;  // the lane's cell
+    const TileDma td{bx * S::TX, by * S::BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
+    const int gi = td.gi0 + col + 4, gj = td.gj0 + row + 4;
     const bool inbox = gi < g.g0 && gj < g.g1;
     const bool active = gi < g.n0 + NG && gj < g.n1 + NG;
     const int kbeg = bz * kz, kend = min(kbeg + kz, g.n2);
@@ -615,12 +659,12 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
     const double *q_u = q + QU * cg, *q_v = q + QV * cg, *q_w = q + QW * cg, *q_t = q + QTEMP * cg;
     const double *const src[DNF] = {q_u, q_v, q_w, q_t, wzf, q_u + 4 * pg, q_v + 4 * pg};
     const int64_t own = (int64_t)gj * xg + gi;
-    const int me = (wave + 4) * PX + lane + 4;  // own cell in 16 x 72 tiles
-    const int me64 = (wave + 4) * TX + lane;    // own cell in s_ux
-    const int meby = wave * PX + lane + 4;      // own cell in s_vy
+    const int me = (row + 4) * S::PX + col + 4;  // own cell in R x PX tiles
+    const int me64 = (row + 4) * S::TX + col;    // own cell in s_ux
+    const int meby = row * S::PX + col + 4;      // own cell in s_vy
     const double dxi0s = dxi0 * dxi0, dxi1s = dxi1 * dxi1, dxi2s = dxi2 * dxi2;
 
-    dma_plane<DNF>(src, (int64_t)(kbeg + NG) * pg, sm, td, wave, lane);
+    dma_plane_s<S, DNF>(src, (int64_t)(kbeg + NG) * pg, sm, td, wave, lane);
 
     // windows: plane k+m at [4+m]; ux and vy of the first 8 planes straight from
     // global memory (the same expressions as the per-plane path)
@@ -661,12 +705,13 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
 
     for (int k = kbeg; k < kend; k++) {
         const int cur = (k - kbeg) & 1;
-        const double *b = sm + cur * (DNF * TILE);
+        const double *b = sm + cur * (DNF * S::TILE);
         plane_barrier();
-        if (k + 1 < kend) dma_plane<DNF>(src, (int64_t)(k + 1 + NG) * pg, sm + (cur ^ 1) * (DNF * TILE), td, wave, lane);
-        const double *s_cu = b + DCU * TILE, *s_cv = b + DCV * TILE, *s_cw = b + DCW * TILE,
-                     *s_ct = b + DCT * TILE, *s_wz = b + DWZ * TILE, *s_lu = b + DLU * TILE,
-                     *s_lv = b + DLV * TILE;
+        if (k + 1 < kend)
+            dma_plane_s<S, DNF>(src, (int64_t)(k + 1 + NG) * pg, sm + (cur ^ 1) * (DNF * S::TILE), td, wave, lane);
+        const double *s_cu = b + DCU * S::TILE, *s_cv = b + DCV * S::TILE, *s_cw = b + DCW * S::TILE,
+                     *s_ct = b + DCT * S::TILE, *s_wz = b + DWZ * S::TILE, *s_lu = b + DLU * S::TILE,
+                     *s_lv = b + DLV * S::TILE;
 
         const int64_t zo = (int64_t)(k + NG) * pg + own;
         wu[8] = s_lu[me];
@@ -682,7 +727,7 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
         // ---------------- phase B
         {
 #define LUX(mm) s_lu[me + (mm)]
-#define LVY(mm) s_lv[me + (mm) * PX]
+#define LVY(mm) s_lv[me + (mm) * S::PX]
             // leading plane: ux, vy of the own column (cpu/diffterm.cpp:77-81, :110-114)
             wux[8] = D1(LUX) * dxi0;
             wvy[8] = D1(LVY) * dxi1;
@@ -692,20 +737,20 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
             // the same values four planes ago), the halo rows 0..3 and 12..15 from the tile
             s_ux[me64] = wux[4];
             {
-                const int rr = threadIdx.x >> 6, r = rr < 4 ? rr : rr + BY, c = threadIdx.x & 63;
-                const int l = r * PX + c + 4;
+                const int rr = threadIdx.x / S::TX, r = rr < 4 ? rr : rr + S::BY, c = threadIdx.x % S::TX;
+                const int l = r * S::PX + c + 4;
 #define CUX(mm) s_cu[l + (mm)]
-                s_ux[r * TX + c] = D1(CUX) * dxi0;
+                s_ux[r * S::TX + c] = D1(CUX) * dxi0;
 #undef CUX
             }
             // centre plane, vy: the own cells from the window, the x-halo columns 0..3 and
             // 68..71 of rows 4..11 from the tile
             s_vy[meby] = wvy[4];
-            if (threadIdx.x < 8 * BY) {
-                const int rr = threadIdx.x >> 3, cc = threadIdx.x & 7, c = cc < 4 ? cc : cc + TX;
-                const int l = (rr + 4) * PX + c;
-#define CVY(mm) s_cv[l + (mm) * PX]
-                s_vy[rr * PX + c] = D1(CVY) * dxi1;
+            if (threadIdx.x < 8 * S::BY) {
+                const int rr = threadIdx.x >> 3, cc = threadIdx.x & 7, c = cc < 4 ? cc : cc + S::TX;
+                const int l = (rr + 4) * S::PX + c;
+#define CVY(mm) s_cv[l + (mm) * S::PX]
+                s_vy[rr * S::PX + c] = D1(CVY) * dxi1;
 #undef CVY
             }
         }
@@ -714,7 +759,7 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
         // ---------------- phase C
         if (active) {
 #define XS(a, mm) a[me + (mm)]
-#define YS(a, mm) a[me + (mm) * PX]
+#define YS(a, mm) a[me + (mm) * S::PX]
 #define CUx(mm) XS(s_cu, mm)
 #define CUy(mm) YS(s_cu, mm)
 #define CUz(mm) wu[4 + (mm)]
@@ -729,7 +774,7 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
 #define CTz(mm) wt[4 + (mm)]
 #define VYx(mm) s_vy[meby + (mm)]
 #define WZx(mm) XS(s_wz, mm)
-#define UXy(mm) s_ux[me64 + (mm) * TX]
+#define UXy(mm) s_ux[me64 + (mm) * S::TX]
 #define WZy(mm) YS(s_wz, mm)
 #define UXz(mm) wux[4 + (mm)]
 #define VYz(mm) wvy[4 + (mm)]
@@ -822,6 +867,65 @@ int pipe_kz(const Geom &g, int tiles_xy, int nbox = 1)
     return kz;
 }
 
+// The step's stencils run on 32 x 8 tiles, two workgroups per CU (512^3 step: diffterm
+// 4.84 -> 4.33 ms, fused hypterm 6.65 -> 6.05 ms against 64 x 8); setting the named
+// variable to 64 selects 64 x 8 tiles (the results are the same bits either way).
+bool wide_tiles(const char *var)
+{
+    const char *e = getenv(var);
+    return e && atoi(e) == 64;
+}
+
+template <class S>
+hipError_t launch_diffterm_shape(const Geom &g, const double *q, const double *wz, double *difflux,
+                                 double dxi0, double dxi1, double dxi2, double eta, double alam,
+                                 hipStream_t s, bool write_rho, const BoxSet &bs)
+{
+    const int ntx = (g.n0 + S::TX - 1) / S::TX, nty = (g.n1 + S::BY - 1) / S::BY;
+    const int kz = pipe_kz(g, ntx * nty, bs.n);
+    const int nbpb = ntx * nty * ((g.n2 + kz - 1) / kz);
+    if (bs.n < 1 || (int64_t)nbpb * bs.n > 0x7fffffff) return hipErrorInvalidValue;
+    const int nb = nbpb * bs.n;
+    if (write_rho)
+        diffterm_pipe_kernel<S, true><<<nb, S::NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz,
+                                                          ntx, nty, bs, nbpb);
+    else
+        diffterm_pipe_kernel<S, false><<<nb, S::NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz,
+                                                           ntx, nty, bs, nbpb);
+    return hipGetLastError();
+}
+
+template <class S>
+hipError_t launch_hyp_cons_shape(const Geom &g, const double *cons, double *flux, double dxi0, double dxi1,
+                                 double dxi2, const double *d, const double *base, double *out,
+                                 const double *dt_dev, int phase, bool write_f, double *qout, int qout_kind,
+                                 hipStream_t s, const BoxSet &bs)
+{
+    const int ntx = (g.n0 + S::TX - 1) / S::TX, nty = (g.n1 + S::BY - 1) / S::BY;
+    const int kz = pipe_kz(g, ntx * nty, bs.n);
+    const int nbpb = ntx * nty * ((g.n2 + kz - 1) / kz);
+    if (bs.n < 1 || (int64_t)nbpb * bs.n > 0x7fffffff) return hipErrorInvalidValue;
+    const int nb = nbpb * bs.n;
+    PipeRk rk{d, base, out, dt_dev, phase, qout};
+    const bool stage1 = phase == 1 && base == cons;
+    if (!qout) qout_kind = 0;
+    if (qout_kind < 0 || qout_kind > 2 || (write_f && qout_kind)) return hipErrorInvalidValue;
+#define HYP_CONS(S1, WF, QO) \
+    hypterm_cons_kernel<S, S1, WF, QO><<<nb, S::NT, 0, s>>>(g, cons, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk, bs, nbpb)
+    // the step's combinations: stage 1 (-> q 1), stage 2 (-> q 2), stage 3 (F, no q);
+    // and no q for any stage
+    if (stage1 && !write_f)
+        qout_kind == 1 ? HYP_CONS(true, false, 1) : (qout_kind == 2 ? HYP_CONS(true, false, 2) : HYP_CONS(true, false, 0));
+    else if (stage1)
+        HYP_CONS(true, true, 0);
+    else if (!write_f)
+        qout_kind == 1 ? HYP_CONS(false, false, 1) : (qout_kind == 2 ? HYP_CONS(false, false, 2) : HYP_CONS(false, false, 0));
+    else
+        HYP_CONS(false, true, 0);
+#undef HYP_CONS
+    return hipGetLastError();
+}
+
 }  // namespace
 
 bool pipe_aligned(const void *p) { return ((uintptr_t)p & 15) == 0; }
@@ -853,30 +957,13 @@ hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, 
                                double *out, const double *dt_dev, int phase, bool write_f,
                                double *qout, int qout_kind, hipStream_t s, const BoxSet &bs)
 {
-    const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
-    const int kz = pipe_kz(g, ntx * nty, bs.n);
-    const int nbpb = ntx * nty * ((g.n2 + kz - 1) / kz);
-    if (bs.n < 1 || (int64_t)nbpb * bs.n > 0x7fffffff) return hipErrorInvalidValue;
-    const int nb = nbpb * bs.n;
-    PipeRk rk{d, base, out, dt_dev, phase, qout};
-    const bool stage1 = phase == 1 && base == cons;
-    if (!qout) qout_kind = 0;
-    if (qout_kind < 0 || qout_kind > 2 || (write_f && qout_kind)) return hipErrorInvalidValue;
-#define HYP_CONS(S1, WF, QO) \
-    hypterm_cons_kernel<S1, WF, QO><<<nb, NT, 0, s>>>(g, cons, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk, bs, nbpb)
-    // the step's combinations: stage 1 (-> q 1), stage 2 (-> q 2), stage 3 (F, no q);
-    // and no q for any stage
-    if (stage1 && !write_f)
-        qout_kind == 1 ? HYP_CONS(true, false, 1) : (qout_kind == 2 ? HYP_CONS(true, false, 2) : HYP_CONS(true, false, 0));
-    else if (stage1)
-        HYP_CONS(true, true, 0);
-    else if (!write_f)
-        qout_kind == 1 ? HYP_CONS(false, false, 1) : (qout_kind == 2 ? HYP_CONS(false, false, 2) : HYP_CONS(false, false, 0));
-    else
-        HYP_CONS(false, true, 0);
-#undef HYP_CONS
-    return hipGetLastError();
+    if (wide_tiles("CNS_HYP_TX"))
+        return launch_hyp_cons_shape<Shape<64>>(g, cons, flux, dxi0, dxi1, dxi2, d, base, out, dt_dev, phase,
+                                                write_f, qout, qout_kind, s, bs);
+    return launch_hyp_cons_shape<Shape<32>>(g, cons, flux, dxi0, dxi1, dxi2, d, base, out, dt_dev, phase,
+                                            write_f, qout, qout_kind, s, bs);
 }
+
 
 hipError_t launch_ctoprim_wz(const Geom &g, const double *u, double *q, double *wz,
                              unsigned long long *acc, double dx0, double dx1, double dx2, double dxi2,
@@ -909,16 +996,9 @@ hipError_t launch_diffterm_pipe(const Geom &g, const double *q, const double *wz
                                 double dxi0, double dxi1, double dxi2, double eta, double alam,
                                 hipStream_t s, bool write_rho, const BoxSet &bs)
 {
-    const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
-    const int kz = pipe_kz(g, ntx * nty, bs.n);
-    const int nbpb = ntx * nty * ((g.n2 + kz - 1) / kz);
-    if (bs.n < 1 || (int64_t)nbpb * bs.n > 0x7fffffff) return hipErrorInvalidValue;
-    const int nb = nbpb * bs.n;
-    if (write_rho)
-        diffterm_pipe_kernel<true><<<nb, NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty, bs, nbpb);
-    else
-        diffterm_pipe_kernel<false><<<nb, NT, 0, s>>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, kz, ntx, nty, bs, nbpb);
-    return hipGetLastError();
+    if (wide_tiles("CNS_DIFF_TX"))
+        return launch_diffterm_shape<Shape<64>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, s, write_rho, bs);
+    return launch_diffterm_shape<Shape<32>>(g, q, wz, difflux, dxi0, dxi1, dxi2, eta, alam, s, write_rho, bs);
 }
 
 }  // namespace cns

@@ -160,7 +160,8 @@ def test_64_cube_100_steps_every_step(orc, device):
 def test_step_compositions_agree(device, monkeypatch, n):
     """Every way the library composes a step leaves every ABI array bit-identical:
     pipelined kernels fused (hypterm + stage update in one pass, with the stage's
-    primitives stored by that pass or by a ctoprim pass per stage) and unfused,
+    primitives stored by that pass or by a ctoprim pass per stage; 32 x 8 or 64 x 8
+    tiles) and unfused,
     register-staged kernels fused and unfused, and the direct kernels.
     Non-cubic boxes go beyond the reference cpu/ path (cubic only), so the direct
     one-thread-per-cell kernels are the cross-check here.  Extents below 8 give cells
@@ -169,14 +170,18 @@ def test_step_compositions_agree(device, monkeypatch, n):
     lay = BoxLayout(n)
     U0 = sinusoid_state(lay, 9, mach=0.2, kmax=3)
     results = {}
-    for name, pipe, fused, impl, stage_q in (("pipe_fused", "1", True, _lib.CNS_IMPL_TILED, "1"),
-                                             ("pipe_fused_ctoprim", "1", True, _lib.CNS_IMPL_TILED, "0"),
-                                             ("pipe_unfused", "1", False, _lib.CNS_IMPL_TILED, "1"),
-                                             ("staged_fused", "0", True, _lib.CNS_IMPL_TILED, "1"),
-                                             ("staged_unfused", "0", False, _lib.CNS_IMPL_TILED, "1"),
-                                             ("direct", "1", False, _lib.CNS_IMPL_DIRECT, "1")):
+    for name, pipe, fused, impl, stage_q, tx in (
+            ("pipe_fused", "1", True, _lib.CNS_IMPL_TILED, "1", "32"),
+            ("pipe_fused_wide_tiles", "1", True, _lib.CNS_IMPL_TILED, "1", "64"),
+            ("pipe_fused_ctoprim", "1", True, _lib.CNS_IMPL_TILED, "0", "32"),
+            ("pipe_unfused", "1", False, _lib.CNS_IMPL_TILED, "1", "32"),
+            ("staged_fused", "0", True, _lib.CNS_IMPL_TILED, "1", "32"),
+            ("staged_unfused", "0", False, _lib.CNS_IMPL_TILED, "1", "32"),
+            ("direct", "1", False, _lib.CNS_IMPL_DIRECT, "1", "32")):
         monkeypatch.setenv("CNS_PIPE", pipe)
         monkeypatch.setenv("CNS_STAGE_Q", stage_q)
+        monkeypatch.setenv("CNS_DIFF_TX", tx)
+        monkeypatch.setenv("CNS_HYP_TX", tx)
         ops.set_stencil_impl(impl)
         try:
             st = ops.Stepper(lay)
