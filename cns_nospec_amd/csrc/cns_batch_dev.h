@@ -47,12 +47,9 @@ hipError_t launch_batch_init(BoxDev *boxes, int box, int e, bool fp32, hipStream
 hipError_t launch_batch_finalize(BoxDev *boxes, const int *box_ids, int nbox, int count_step,
                                  hipStream_t s);
 // One RK stage over the tile list: the list holds class 0, 1, 2 tiles in the
-// ranges [start[c], start[c]+count[c]); grid[c] caps class c's persistent diffterm
-// launch and grid[3 * stage + c] its hypterm launch of that stage.
+// ranges [start[c], start[c]+count[c]); each class gets a diffterm launch, then a
+// hypterm + update launch, of one workgroup per tile.
 hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int start[3],
-                              const int count[3], const int grid[12], int stage, bool fp32, hipStream_t s);
-// Resident-workgroup capacity (CUs x blocks per CU) of each class's diffterm kernel
-// (grid[c]) and stage-s hypterm kernel (grid[3 * s + c]).
-void batch_class_grids(bool fp32, int grid[12]);
+                              const int count[3], int stage, bool fp32, hipStream_t s);
 
 }  // namespace cns

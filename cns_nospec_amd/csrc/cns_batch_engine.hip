@@ -84,7 +84,8 @@ struct BoxHost {
 int width_class(int w) { return w == 16 ? 0 : (w == 32 ? 1 : (w == 64 ? 2 : -1)); }
 
 // Default tile: the width wasting the fewest lanes on an e x e plane (ties to the
-// wider tile), z runs of 32 planes.
+// wider tile), z runs of 16 planes (on the mixed request stream runs of 12-16 planes
+// step 6 % faster than runs of 32 or of the whole box, and 4 % faster than runs of 8).
 std::pair<int, int> default_tile(int e)
 {
     int best = 64;
@@ -97,7 +98,7 @@ std::pair<int, int> default_tile(int e)
             best = w;
         }
     }
-    return {best, std::min(e, 32)};
+    return {best, std::min(e, 16)};
 }
 
 }  // namespace
@@ -119,7 +120,6 @@ struct cns_batch {
     std::vector<int> hids;
     std::map<int, std::pair<int, int>> tune;  // edge -> (width, kz)
     int cls_start[kTileClasses] = {0, 0, 0}, cls_count[kTileClasses] = {0, 0, 0};
-    int grid[4 * kTileClasses] = {};  // diffterm, then the hypterm launches of stages 1..3
     int nresident = 0, ntiles = 0;
     bool dirty = true;
     hipStream_t cap_stream = nullptr;
@@ -210,8 +210,8 @@ int rebuild(cns_batch *b, hipStream_t s)
         CNS_HIPCHK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
         for (int stage = 1; stage <= 3 && e == hipSuccess; stage++)
-            e = launch_batch_stage(b->d_boxes, b->d_tiles, b->cls_start, b->cls_count, b->grid, stage,
-                                   b->fp32, b->cap_stream);
+            e = launch_batch_stage(b->d_boxes, b->d_tiles, b->cls_start, b->cls_count, stage, b->fp32,
+                                   b->cap_stream);
         if (e == hipSuccess) e = launch_batch_finalize(b->d_boxes, b->d_ids, b->nresident, 1, b->cap_stream);
         hipGraph_t g = nullptr;
         const hipError_t e2 = hipStreamEndCapture(b->cap_stream, &g);
@@ -277,7 +277,6 @@ int cns_batch_create(cns_batch **out, size_t hbm_budget, int flux_fp32, int max_
     }
     b->pool.init(b->pool_mem, b->pool_mem ? hbm_budget : 0);
     b->hrec.resize(max_boxes);
-    batch_class_grids(b->fp32, b->grid);
     *out = b;
     return CNS_OK;
 }

@@ -27,15 +27,6 @@ namespace cns {
 
 namespace {
 
-constexpr int TX = 64;              // tile width (one wave)
-constexpr int PX = TX + 8;          // LDS pitch (x halos)
-constexpr int BY = 8;               // tile height (waves per workgroup)
-constexpr int R = BY + 8;           // LDS rows (y halos)
-constexpr int NT = TX * BY;         // 512 threads
-constexpr int TILE = R * PX;        // 1152 doubles per staged field
-constexpr int TILE_DMA = TILE / 128;  // 9 wave-wide dwordx4 loads per field
-static_assert(TILE % 128 == 0 && TILE_DMA == 9, "tile must be a whole number of 1 KB DMA blocks");
-
 typedef __attribute__((address_space(3))) void lds_void;
 
 #define D1(F) (ALP * (F(1) - F(-1)) + BET * (F(2) - F(-2)) + GAM * (F(3) - F(-3)) + DEL * (F(4) - F(-4)))
@@ -71,35 +62,6 @@ struct TileDma {
     unsigned xg;
 };
 
-// DMA block n (0..8) of one field tile: lane L moves tile elements e, e+1 with
-// e = 128 n + 2 L (row e / 72, column e % 72; 72 is even, so a pair never straddles
-// rows) from the plane at `src` into tile + 128 n + 2 L.
-__device__ __forceinline__ void dma_block(const double *src, double *tile, int n, const TileDma &t,
-                                          int lane)
-{
-    const int e = 128 * n + 2 * lane;
-    const int r = e / PX, c = e - r * PX;
-    const int hj = min(t.gj0 + r, t.g1m1), hi = min(t.gi0 + c, t.xgm2);
-    // a plane offset fits 32 bits; keeping it 32-bit halves what the compiler holds
-    // live across the plane loop
-    const unsigned off = (unsigned)hj * t.xg + (unsigned)hi;
-    __builtin_amdgcn_global_load_lds((const void *)(src + off), (lds_void *)(tile + 128 * n), 16, 0, 0);
-}
-
-// This wave's share of the 9*NF DMA blocks of NF field tiles of one plane: block
-// p = 9 f + n goes to wave p % 8, which balances 8 or 7 blocks per wave for NF = 7.
-template <int NF>
-__device__ __forceinline__ void dma_plane(const double *const (&src)[NF], int64_t plane,
-                                          double *buf, const TileDma &t, int wave, int lane)
-{
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-        const int n0 = (wave - f) & 7;
-        dma_block(src[f] + plane, buf + f * TILE, n0, t, lane);
-        if (n0 == 0) dma_block(src[f] + plane, buf + f * TILE, 8, t, lane);
-    }
-}
-
 // All of this wave's loads, DMA included, have landed; then the workgroup barrier.
 __device__ __forceinline__ void plane_barrier()
 {
@@ -119,6 +81,10 @@ struct Shape {
     static_assert(PX % 2 == 0, "a lane's 16-byte pair stays in one row");
 };
 
+// DMA block n of one field tile: lane L moves tile elements e, e+1 with e = 128 n + 2 L
+// (row e / PX, column e % PX; PX is even, so a pair never straddles rows) from the
+// plane at `src` into tile + 128 n + 2 L.  A plane offset fits 32 bits; keeping it
+// 32-bit halves what the compiler holds live across the plane loop.
 template <class S>
 __device__ __forceinline__ void dma_block_s(const double *src, double *tile, int n, const TileDma &t, int lane)
 {
@@ -152,10 +118,10 @@ struct PipeRk {
     double *qout;        // primitives of the result (interior cells), or null
 };
 
-// hypterm's flux of the lane's cell (LDS index me of 16 x 72 tiles): x terms from the
+// hypterm's flux of the lane's cell (LDS index me of 16 x P tiles): x terms from the
 // row, y terms from the column, z terms from the register windows (plane k+m at
 // [4+m]), in the reference's order (cpu/hypterm.cpp:33-90, :101-147, :158-204).
-template <int P = PX>
+template <int P>
 __device__ __forceinline__ void hyp_flux(const double *s_mx, const double *s_my, const double *s_mz,
                                          const double *s_E, const double *s_p, const double *s_u,
                                          const double *s_v, int me, const double (&wmx)[9],
@@ -279,19 +245,21 @@ __device__ __forceinline__ void q_store(const double (&o)[5], double *q, int64_t
 //    stage overwrites).
 enum { HMX = 0, HMY, HMZ, HE, HP, HU, HV, HNF };
 
-template <bool RK, bool STAGE1, bool WRITE_F>
-__global__ void __launch_bounds__(NT, 2)
+template <class S, bool RK, bool STAGE1, bool WRITE_F>
+__global__ void __launch_bounds__(S::NT, 2)
 hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__restrict__ q,
                     double *__restrict__ flux, double dxi0, double dxi1, double dxi2, int kz,
                     int ntx, int nty, PipeRk rk)
 {
-    __shared__ double sm[2 * HNF * TILE];
+    constexpr int TILE_ = S::TILE, PX_ = S::PX;
+    __shared__ double sm[2 * HNF * TILE_];
 
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const TileDma td{bx * TX, by * BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
-    const int gi = td.gi0 + lane + 4, gj = td.gj0 + wave + 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;       // DMA issue
+    const int col = threadIdx.x % S::TX, row = threadIdx.x / S::TX;  // the lane's cell
+    const TileDma td{bx * S::TX, by * S::BY, g.g1 - 1, g.xg - 2, (unsigned)g.xg};
+    const int gi = td.gi0 + col + 4, gj = td.gj0 + row + 4;
     const bool inbox = gi < g.g0 && gj < g.g1;
     const bool active = gi < g.n0 + NG && gj < g.n1 + NG;
     const int kbeg = bz * kz, kend = min(kbeg + kz, g.n2);
@@ -301,7 +269,7 @@ hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__res
     const double *q_u = q + QU * cg, *q_v = q + QV * cg, *q_w = q + QW * cg, *q_p = q + QPRES * cg;
     const double *const src[HNF] = {c_mx, c_my, c_mz, c_E, q_p, q_u, q_v};
     const int64_t own = (int64_t)gj * g.xg + gi;
-    const int me = (wave + 4) * PX + lane + 4;
+    const int me = (row + 4) * PX_ + col + 4;
     double dt = 0.0, ra = 0.0, rb = 1.0;
     if (RK) {
         // r = ra*base + rb*(cons + dt*(D+F)) with (ra, rb) = (0, 1), (3/4, 1/4), (1/3, 2/3):
@@ -313,7 +281,7 @@ hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__res
     }
 
     // first plane's tiles in flight while the windows fill
-    dma_plane<HNF>(src, (int64_t)(kbeg + NG) * pg, sm, td, wave, lane);
+    dma_plane_s<S, HNF>(src, (int64_t)(kbeg + NG) * pg, sm, td, wave, lane);
 
     double wmx[9], wmy[9], wmz[9], wE[9], ww[9], wp[9];
     double nmx = 0, nmy = 0, nmz = 0, nE = 0, nw = 0, np = 0;
@@ -330,9 +298,10 @@ hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__res
 
     for (int k = kbeg; k < kend; k++) {
         const int cur = (k - kbeg) & 1;
-        const double *b = sm + cur * (HNF * TILE);
+        const double *b = sm + cur * (HNF * TILE_);
         plane_barrier();
-        if (k + 1 < kend) dma_plane<HNF>(src, (int64_t)(k + 1 + NG) * pg, sm + (cur ^ 1) * (HNF * TILE), td, wave, lane);
+        if (k + 1 < kend)
+            dma_plane_s<S, HNF>(src, (int64_t)(k + 1 + NG) * pg, sm + (cur ^ 1) * (HNF * TILE_), td, wave, lane);
 
         const int64_t zo = (int64_t)(k + NG) * pg + own;
         wmx[8] = nmx; wmy[8] = nmy; wmz[8] = nmz; wE[8] = nE; ww[8] = nw; wp[8] = np;
@@ -353,11 +322,11 @@ hypterm_pipe_kernel(Geom g, const double *__restrict__ cons, const double *__res
         }
 
         if (active) {
-            const double *s_mx = b + HMX * TILE, *s_my = b + HMY * TILE, *s_mz = b + HMZ * TILE,
-                         *s_E = b + HE * TILE, *s_p = b + HP * TILE, *s_u = b + HU * TILE,
-                         *s_v = b + HV * TILE;
+            const double *s_mx = b + HMX * TILE_, *s_my = b + HMY * TILE_, *s_mz = b + HMZ * TILE_,
+                         *s_E = b + HE * TILE_, *s_p = b + HP * TILE_, *s_u = b + HU * TILE_,
+                         *s_v = b + HV * TILE_;
             double F[5];
-            hyp_flux(s_mx, s_my, s_mz, s_E, s_p, s_u, s_v, me, wmx, wmy, wmz, wE, ww, wp, dxi0, dxi1, dxi2, F);
+            hyp_flux<PX_>(s_mx, s_my, s_mz, s_E, s_p, s_u, s_v, me, wmx, wmy, wmz, wE, ww, wp, dxi0, dxi1, dxi2, F);
             if (RK) {
                 const double uc[5] = {rho_c, wmx[4], wmy[4], wmz[4], wE[4]};
                 double o[5];
@@ -877,6 +846,29 @@ bool wide_tiles(const char *var)
 }
 
 template <class S>
+hipError_t launch_hyp_pipe_shape(const Geom &g, const double *cons, const double *q, double *flux, double dxi0,
+                                 double dxi1, double dxi2, const double *d, const double *base, double *out,
+                                 const double *dt_dev, int phase, bool write_f, hipStream_t s)
+{
+    const int ntx = (g.n0 + S::TX - 1) / S::TX, nty = (g.n1 + S::BY - 1) / S::BY;
+    const int kz = pipe_kz(g, ntx * nty);
+    const int nb = ntx * nty * ((g.n2 + kz - 1) / kz);
+    PipeRk rk{d, base, out, dt_dev, phase};
+    const bool stage1 = phase == 1 && base == cons;
+#define HYP_PIPE(RK, S1, WF)                                                                              \
+    hypterm_pipe_kernel<S, RK, S1, WF><<<nb, S::NT, 0, s>>>(g, cons, q, flux, dxi0, dxi1, dxi2, kz, ntx, nty, \
+                                                            rk)
+    if (!phase)
+        HYP_PIPE(false, false, true);
+    else if (stage1)
+        write_f ? HYP_PIPE(true, true, true) : HYP_PIPE(true, true, false);
+    else
+        write_f ? HYP_PIPE(true, false, true) : HYP_PIPE(true, false, false);
+#undef HYP_PIPE
+    return hipGetLastError();
+}
+
+template <class S>
 hipError_t launch_diffterm_shape(const Geom &g, const double *q, const double *wz, double *difflux,
                                  double dxi0, double dxi1, double dxi2, double eta, double alam,
                                  hipStream_t s, bool write_rho, const BoxSet &bs)
@@ -935,21 +927,11 @@ hipError_t launch_hypterm_pipe(const Geom &g, const double *cons, const double *
                                const double *base, double *out, const double *dt_dev, int phase,
                                bool write_f, hipStream_t s)
 {
-    const int ntx = (g.n0 + TX - 1) / TX, nty = (g.n1 + BY - 1) / BY;
-    const int kz = pipe_kz(g, ntx * nty);
-    const int nb = ntx * nty * ((g.n2 + kz - 1) / kz);
-    PipeRk rk{d, base, out, dt_dev, phase};
-    const bool stage1 = phase == 1 && base == cons;
-#define HYP_PIPE(RK, S1, WF) \
-    hypterm_pipe_kernel<RK, S1, WF><<<nb, NT, 0, s>>>(g, cons, q, flux, dxi0, dxi1, dxi2, kz, ntx, nty, rk)
-    if (!phase)
-        HYP_PIPE(false, false, true);
-    else if (stage1)
-        write_f ? HYP_PIPE(true, true, true) : HYP_PIPE(true, true, false);
-    else
-        write_f ? HYP_PIPE(true, false, true) : HYP_PIPE(true, false, false);
-#undef HYP_PIPE
-    return hipGetLastError();
+    if (wide_tiles("CNS_HYP_TX"))
+        return launch_hyp_pipe_shape<Shape<64>>(g, cons, q, flux, dxi0, dxi1, dxi2, d, base, out, dt_dev, phase,
+                                                write_f, s);
+    return launch_hyp_pipe_shape<Shape<32>>(g, cons, q, flux, dxi0, dxi1, dxi2, d, base, out, dt_dev, phase,
+                                            write_f, s);
 }
 
 hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, double dxi0,

@@ -11,7 +11,7 @@
  * e^3 doubles each, index (k*e + j)*e + i, component outermost (the periodic
  * wrap is done by index).  A step runs, per RK stage, a diffterm launch and a
  * hypterm + RK-update launch over a tile work list of every active box (one
- * persistent launch per tile-width class), then a dt finalize; it is captured
+ * launch per tile-width class, a workgroup per tile), then a dt finalize; it is captured
  * once into a hipGraph and replayed.  State, RK3 accumulation, p/T recovery and
  * the dt reduction are always FP64; with flux_fp32 the stencil arithmetic of
  * hypterm and diffterm runs in FP32 (the FP64 engine is bit-identical to
