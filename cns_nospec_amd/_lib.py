@@ -108,6 +108,8 @@ _SIGS = {
     "cns_read_field": (_INT, [C.c_char_p, _I3, _INT, _INT, _VP]),
     "cns_write_field": (_INT, [C.c_char_p, _I3, _INT, _INT, _VP]),
     "cns_field_len": (C.c_size_t, [_I3, _INT]),
+    "cns_write_plotfile": (_INT, [C.c_char_p, _INT, _INT, _VP, _VP, _INT, _VP, _D3, _D3, _D3,
+                                  C.c_double]),
 }
 
 PROF_KINDS = ("fill", "ctoprim", "diffterm", "hypterm", "rk", "hypterm_rk", "fill_ctoprim", "other")

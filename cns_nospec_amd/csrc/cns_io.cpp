@@ -14,6 +14,8 @@
 #include <thread>
 #include <vector>
 
+#include <sys/stat.h>
+
 #include "cns_host.h"
 #include "cns_io.h"
 
@@ -290,6 +292,133 @@ int cns_write_field(const char *path, const int dim_g[3], int ncomp, int header,
     }
     err = err || std::ferror(f);
     if (std::fclose(f) != 0 || err) return fail(CNS_EIO, "write error on %s", path);
+    return CNS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int make_dir(const std::string &d)
+{
+    if (mkdir(d.c_str(), 0755) == 0 || errno == EEXIST) return CNS_OK;
+    return fail(CNS_EIO, "cannot create directory %s: %s", d.c_str(), std::strerror(errno));
+}
+
+}  // namespace
+
+extern "C" {
+
+int cns_write_plotfile(const char *dir, int ncomp, int nbox, const int *lo, const int *hi, int ng,
+                       const double *host, const double prob_lo[3], const double prob_hi[3],
+                       const double dx[3], double time)
+{
+    if (!dir || !*dir || ncomp <= 0 || ncomp > 99 || nbox <= 0 || !lo || !hi || ng < 0 || !host ||
+        !prob_lo || !prob_hi || !dx)
+        return fail(CNS_EINVAL, "bad argument (dir, ncomp, nbox, boxes, ng, host or geometry)");
+    int dlo[3], dhi[3];
+    std::vector<size_t> offs(size_t(nbox) + 1, 0);  // each box's start in host
+    for (int b = 0; b < nbox; b++) {
+        int dg[3];
+        for (int d = 0; d < 3; d++) {
+            if (hi[3 * b + d] < lo[3 * b + d]) return fail(CNS_EINVAL, "box %d is empty along %d", b, d);
+            dg[d] = hi[3 * b + d] - lo[3 * b + d] + 1 + 2 * ng;
+            dlo[d] = b ? std::min(dlo[d], lo[3 * b + d]) : lo[3 * b + d];
+            dhi[d] = b ? std::max(dhi[d], hi[3 * b + d]) : hi[3 * b + d];
+        }
+        offs[b + 1] = offs[b] + cns_field_len(dg, ncomp);
+    }
+    const std::string root(dir), lev = root + "/Level_00";
+    int rc = make_dir(root);
+    if (rc == CNS_OK) rc = make_dir(lev);
+    if (rc != CNS_OK) return rc;
+
+    // Cell_D_00000: every box's FAB; the valid cells only, component by component
+    std::vector<long long> fab_at(static_cast<size_t>(nbox));
+    std::vector<double> mn(size_t(nbox) * ncomp), mx(size_t(nbox) * ncomp);
+    {
+        const std::string path = lev + "/Cell_D_00000";
+        FILE *f = std::fopen(path.c_str(), "wb");
+        if (!f) return fail(CNS_EIO, "cannot open %s: %s", path.c_str(), std::strerror(errno));
+        bool err = false;
+        std::vector<double> buf;
+        for (int b = 0; b < nbox && !err; b++) {
+            const int *l = lo + 3 * b, *h = hi + 3 * b;
+            const int n0 = h[0] - l[0] + 1, n1 = h[1] - l[1] + 1, n2 = h[2] - l[2] + 1;
+            const int dg[3] = {n0 + 2 * ng, n1 + 2 * ng, n2 + 2 * ng};
+            Padded P;
+            padded_of(dg, P);
+            fab_at[b] = std::ftell(f);
+            err = std::fprintf(f,
+                               "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))"
+                               "((%d,%d,%d) (%d,%d,%d) (0,0,0)) %d\n",
+                               l[0], l[1], l[2], h[0], h[1], h[2], ncomp) < 0;
+            buf.resize(size_t(n0) * n1 * n2);
+            for (int c = 0; c < ncomp && !err; c++) {
+                const double *src = host + offs[b] + size_t(c) * P.cg;
+                size_t at = 0;
+                double lo_v = HUGE_VAL, hi_v = -HUGE_VAL;
+                for (int k = 0; k < n2; k++)
+                    for (int j = 0; j < n1; j++) {
+                        const double *row = src + size_t(k + ng) * P.pg + size_t(j + ng) * P.xg + ng;
+                        for (int i = 0; i < n0; i++) {
+                            const double v = row[i];
+                            buf[at++] = v;
+                            lo_v = std::min(lo_v, v);
+                            hi_v = std::max(hi_v, v);
+                        }
+                    }
+                mn[size_t(b) * ncomp + c] = lo_v;
+                mx[size_t(b) * ncomp + c] = hi_v;
+                // x86-64 and the GPU hosts are little-endian, as the FAB header declares
+                err = std::fwrite(buf.data(), sizeof(double), buf.size(), f) != buf.size();
+            }
+        }
+        err = err || std::ferror(f);
+        if (std::fclose(f) != 0 || err) return fail(CNS_EIO, "write error on %s", path.c_str());
+    }
+    // Cell_H: the FabArray header
+    {
+        const std::string path = lev + "/Cell_H";
+        FILE *f = std::fopen(path.c_str(), "w");
+        if (!f) return fail(CNS_EIO, "cannot open %s: %s", path.c_str(), std::strerror(errno));
+        std::fprintf(f, "1\n1\n%d\n0\n(%d 0\n", ncomp, nbox);
+        for (int b = 0; b < nbox; b++)
+            std::fprintf(f, "((%d,%d,%d) (%d,%d,%d) (0,0,0))\n", lo[3 * b], lo[3 * b + 1], lo[3 * b + 2],
+                         hi[3 * b], hi[3 * b + 1], hi[3 * b + 2]);
+        std::fprintf(f, ")\n%d\n", nbox);
+        for (int b = 0; b < nbox; b++) std::fprintf(f, "FabOnDisk: Cell_D_00000 %lld\n", fab_at[b]);
+        for (const std::vector<double> *v : {&mn, &mx}) {
+            std::fprintf(f, "\n%d,%d\n", nbox, ncomp);
+            for (int b = 0; b < nbox; b++) {
+                for (int c = 0; c < ncomp; c++) std::fprintf(f, "%.17e,", (*v)[size_t(b) * ncomp + c]);
+                std::fprintf(f, "\n");
+            }
+        }
+        const bool err = std::ferror(f);
+        if (std::fclose(f) != 0 || err) return fail(CNS_EIO, "write error on %s", path.c_str());
+    }
+    // Header: the plotfile's own header
+    {
+        const std::string path = root + "/Header";
+        FILE *f = std::fopen(path.c_str(), "w");
+        if (!f) return fail(CNS_EIO, "cannot open %s: %s", path.c_str(), std::strerror(errno));
+        std::fprintf(f, "NavierStokes-V1.1\n%d\n", ncomp);
+        for (int c = 0; c < ncomp; c++) std::fprintf(f, "Variable_%02d\n", c + 1);
+        std::fprintf(f, "3\n%.17e\n0\n", time);
+        std::fprintf(f, "%.17e %.17e %.17e\n", prob_lo[0], prob_lo[1], prob_lo[2]);
+        std::fprintf(f, "%.17e %.17e %.17e\n", prob_hi[0], prob_hi[1], prob_hi[2]);
+        std::fprintf(f, "\n((%d,%d,%d) (%d,%d,%d) (0,0,0))\n0\n", dlo[0], dlo[1], dlo[2], dhi[0], dhi[1], dhi[2]);
+        std::fprintf(f, "%.17e %.17e %.17e\n0\n0\n", dx[0], dx[1], dx[2]);
+        std::fprintf(f, "0 %d %.17e\n0\n", nbox, time);
+        for (int b = 0; b < nbox; b++)
+            for (int d = 0; d < 3; d++)
+                std::fprintf(f, "%.17e %.17e\n", prob_lo[d] + dx[d] * lo[3 * b + d],
+                             prob_lo[d] + dx[d] * (hi[3 * b + d] + 1));
+        std::fprintf(f, "Level_00/Cell\n");
+        const bool err = std::ferror(f);
+        if (std::fclose(f) != 0 || err) return fail(CNS_EIO, "write error on %s", path.c_str());
+    }
     return CNS_OK;
 }
 

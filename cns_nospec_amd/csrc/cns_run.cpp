@@ -1,6 +1,6 @@
 // cns_run -- the reference program's main loop on libcns_hip.so.
 //
-//   cns_run [-d testcases_dir] [-o output] [-n nsteps] [-q] [name]
+//   cns_run [-d testcases_dir] [-o output] [-n nsteps] [-q] [-p plot_int [-P plot_dir]] [name]
 //
 // Reads <dir>/general_input and <dir>/multistep_input (or <dir>/<name>_general_input
 // and <dir>/<name>_multistep_input when a name is given; dir defaults to
@@ -9,6 +9,13 @@
 // and "Total time: %lf", and writes the ghosted U to `output` in the reference's
 // format.  Follows gpu/main.cu:11-68 and cpu/main.cpp:9-58; the timed region, as in
 // gpu/main.cu:44-52, spans the copy in, the steps and the copy out.
+//
+// With -p plot_int > 0 it also writes BoxLib plotfiles of U as the Fortran driver
+// does (fortran90/main.f90:122-125, :163-168): plot_dir/plt00000 before the first
+// step, then plt%05d after every step that is a multiple of plot_int and after the
+// last, with time = the sum of the steps' dt and the domain centred on the origin
+// (prob_lo = -n dx / 2, prob_hi = n dx / 2: -0.1 and 0.1 for the reference's inputs,
+// fortran90/main.f90:93-106).  Writing them is not part of the timed region.
 //
 // Exit codes: 0 done, 1 usage / input / output error, 2 library or HIP error,
 // 3 the state became non-physical (non-finite Courant number).
@@ -36,7 +43,8 @@ double now_s()
 
 int usage(const char *argv0)
 {
-    std::fprintf(stderr, "usage: %s [-d testcases_dir] [-o output] [-n nsteps] [-q] [name]\n", argv0);
+    std::fprintf(stderr, "usage: %s [-d testcases_dir] [-o output] [-n nsteps] [-q] [-p plot_int [-P plot_dir]] [name]\n",
+                 argv0);
     return 1;
 }
 
@@ -64,15 +72,17 @@ struct DeviceBuffers {
 
 int main(int argc, char **argv)
 {
-    std::string dir = "../testcases", out = "output", name;
-    int nsteps_override = -1;
+    std::string dir = "../testcases", out = "output", name, plot_dir = ".";
+    int nsteps_override = -1, plot_int = 0;
     bool quiet = false;
     for (int a = 1; a < argc; a++) {
         std::string s = argv[a];
-        if ((s == "-d" || s == "-o" || s == "-n") && a + 1 < argc) {
+        if ((s == "-d" || s == "-o" || s == "-n" || s == "-p" || s == "-P") && a + 1 < argc) {
             std::string v = argv[++a];
             if (s == "-d") dir = v;
             else if (s == "-o") out = v;
+            else if (s == "-P") plot_dir = v;
+            else if (s == "-p") plot_int = std::atoi(v.c_str());
             else nsteps_override = std::atoi(v.c_str());
         } else if (s == "-q") {
             quiet = true;
@@ -119,10 +129,35 @@ int main(int argc, char **argv)
         return 2;
     }
 
+    // plotfile of the device U at step istep (fortran90/write_plotfile.f90:11-53)
+    double plot_s = 0.0, time = 0.0;
+    auto plot = [&](int istep) -> int {
+        const double t0 = now_s();
+        if (hipMemcpy(U.data(), dev.p[0], lenU * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+            std::fprintf(stderr, "cns_run: copying U for plotfile %d failed\n", istep);
+            return 2;
+        }
+        char nm[32];
+        std::snprintf(nm, sizeof nm, "/plt%05d", istep);
+        double plo[3], phi[3];
+        for (int d = 0; d < 3; d++) {
+            phi[d] = 0.5 * cfg.dim[d] * cfg.dx[d];
+            plo[d] = -phi[d];
+        }
+        if (cns_write_plotfile((plot_dir + nm).c_str(), 5, 1, cfg.lo, cfg.hi, cfg.ng, U.data(), plo, phi,
+                               cfg.dx, time) != CNS_OK) {
+            std::fprintf(stderr, "cns_run: %s\n", cns_last_error_string());
+            return 1;
+        }
+        plot_s += now_s() - t0;
+        return 0;
+    };
+
     int rc = 0;
     double total = -now_s();
     HIP_OR_DIE(hipMemcpy(dev.p[0], U.data(), lenU * sizeof(double), hipMemcpyHostToDevice));
-    for (int s = 0; s < cfg.nsteps; s++) {
+    if (plot_int > 0) rc = plot(0);
+    for (int s = 0; s < cfg.nsteps && rc == 0; s++) {
         if (cns_advance_async(ctx, dev.p[0], dev.p[1], dev.p[2], dev.p[3], dev.p[4], dev.scalars,
                               dev.scalars + 1) != CNS_OK) {
             std::fprintf(stderr, "cns_run: step %d: %s\n", s + 1, cns_last_error_string());
@@ -137,9 +172,11 @@ int main(int argc, char **argv)
             rc = 3;
             break;
         }
+        time += h[0];
+        if (plot_int > 0 && ((s + 1) % plot_int == 0 || s + 1 == cfg.nsteps)) rc = plot(s + 1);
     }
     if (rc == 0) HIP_OR_DIE(hipMemcpy(U.data(), dev.p[0], lenU * sizeof(double), hipMemcpyDeviceToHost));
-    total += now_s();
+    total += now_s() - plot_s;
     cns_ctx_destroy(ctx);
     if (rc) return rc;
     std::printf("Total time: %lf\n", total);

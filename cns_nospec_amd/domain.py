@@ -107,6 +107,14 @@ class Domain:
         lib = _stream()
         _lib.check(lib.cns_domain_fill_boundary(self._h, _ptr(U, self.array_len(NC, True), "U")))
 
+    def write_plotfile(self, dirname: str, U, time: float, ncomp: int = NC):
+        """BoxLib plotfile of a ghosted domain array, one FAB per box, as the Fortran
+        driver writes it (fortran90/write_plotfile.f90:11-53; include/cns_io.h)."""
+        from .refio import write_plotfile
+        host = U.detach().cpu().numpy() if hasattr(U, "detach") else U
+        write_plotfile(dirname, host, [self.box(n) for n in range(self.nbox)], self.dx, time,
+                       ncomp=ncomp, ng=self.box_layout.ng)
+
     def advance(self, U, Unew, Q, D, F) -> float:
         """One SSP-RK3 step of every box (fortran90/advance.f90:57-410); returns dt."""
         lib = _stream()

@@ -72,6 +72,27 @@ int cns_write_field(const char *path, const int dim_g[3], int ncomp, int header,
  * No counterpart in the reference. */
 size_t cns_field_len(const int dim_g[3], int ncomp);
 
+/* Writes the directory `dir` as a BoxLib plotfile of one level: the valid (interior)
+ * cells of nbox boxes, ncomp components named Variable_01, Variable_02, ...
+ * Box b covers the 0-based cells lo[3b..3b+2] .. hi[3b..3b+2] of the domain (their
+ * bounding box is the problem domain); `host` holds the boxes back to back, box b as a
+ * padded ghosted array of its extents with ng ghost cells (the layout of include/cns.h;
+ * nbox = 1 is a single box, a cns_domain array is nbox equal boxes).  prob_lo/prob_hi
+ * are the physical corners of the domain, dx the cell size, time the solution time.
+ * Files: dir/Header (format "NavierStokes-V1.1": names, dimension, time, finest level 0,
+ * prob_lo, prob_hi, problem domain, dx, each box's physical extent, "Level_00/Cell"),
+ * dir/Level_00/Cell_H (the box array, the FAB offsets, per-box per-component min and
+ * max) and dir/Level_00/Cell_D_00000 (per box a "FAB ((8, (64 11 52 0 1 12 0 1023)),
+ * (8, (8 7 6 5 4 3 2 1)))((lo) (hi) (0,0,0)) ncomp" line, then its ncomp components of
+ * little-endian doubles, x fastest).  dir is created if absent.
+ * Replaces: write_plotfile  fortran90/write_plotfile.f90:11-53 (U at step istep into
+ * "plt%05d", fortran90/main.f90:122-125, :163-168), whose file writer is BoxLib's
+ * fabio_ml_multifab_write_d (not in the reference tree: the layout above follows the
+ * published BoxLib plotfile format and is not checked byte for byte). */
+int cns_write_plotfile(const char *dir, int ncomp, int nbox, const int *lo, const int *hi, int ng,
+                       const double *host, const double prob_lo[3], const double prob_hi[3],
+                       const double dx[3], double time);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,6 +112,30 @@ def test_domain_step_equals_single_box(device, ncell, m):
     dom.close()
 
 
+def test_domain_plotfile(device, tmp_path):
+    """Domain.write_plotfile: one FAB per box; reassembled, the boxes are the domain's
+    state (gathered) bit for bit."""
+    import torch
+    from cns_nospec_amd import refio
+    ncell, m = (24, 16, 8), 8
+    dom = Domain(ncell, m)
+    U, Unew, Q, D, F = dom.allocate(device)
+    lay = BoxLayout(ncell)
+    U0 = sinusoid_state(lay, 11, mach=0.2, kmax=2)
+    dom.scatter(torch.from_numpy(np.ascontiguousarray(lay.interior_of(lay.gview(U0, NC)))).to(device), NC, U)
+    dt = dom.advance(U, Unew, Q, D, F)
+    d = str(tmp_path / "plt00001")
+    dom.write_plotfile(d, U, dt)
+    rd = refio.read_plotfile(d)
+    whole = dom.gather(U, NC).cpu().numpy()
+    assert rd["time"] == dt and len(rd["boxes"]) == dom.nbox
+    got = np.full_like(whole, np.nan)
+    for (lo, hi), data in zip(rd["boxes"], rd["data"]):
+        got[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = data
+    assert np.array_equal(got, whole)
+    dom.close()
+
+
 def test_domain_nonphysical_and_alignment(device):
     import torch
     dom = Domain((16, 16, 16), 8)

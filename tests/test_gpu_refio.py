@@ -88,3 +88,25 @@ def test_driver_nonphysical_state(tmp_path, device):
     assert r.returncode == 3, (r.returncode, r.stderr)
     assert "not finite" in r.stderr
     assert not os.path.exists(tmp_path / "output")
+
+
+def test_driver_plotfiles(tmp_path, device):
+    """-p plot_int: plt00000 holds the input state, plt00002 / plt00004 and the last step
+    (plt00005) are written as fortran90/main.f90:122-168 does; the last equals the
+    output file's interior bit for bit, with time = the sum of the printed dts."""
+    lay, U = _state_from(CASES["sin16"])
+    cfg = refio.make_config(lay.n, box_dx(lay.n), CFL, ETA, ALAM, 5)
+    refio.write_testcase(str(tmp_path / "testcases"), cfg, U)
+    plots = tmp_path / "plots"
+    plots.mkdir()
+    r = _run(tmp_path, plot_int=2, plot_dir=str(plots))
+    assert sorted(os.listdir(plots)) == ["plt00000", "plt00002", "plt00004", "plt00005"]
+    first = refio.read_plotfile(str(plots / "plt00000"))
+    assert first["time"] == 0.0
+    assert np.array_equal(first["data"][0], lay.interior_of(lay.gview(U, NC)))
+    last = refio.read_plotfile(str(plots / "plt00005"))
+    out = refio.read_field(str(tmp_path / "output"), lay.dimg, NC, header=True)
+    assert np.array_equal(last["data"][0], lay.interior_of(lay.gview(out, NC)))
+    dts = [float(ln.split("=")[1].split(",")[0]) for ln in refio.step_lines(r.stdout)]
+    assert abs(last["time"] - sum(dts)) <= 1e-6 * sum(dts)  # printed dts carry 7 digits
+    assert last["boxes"] == [((0, 0, 0), tuple(v - 1 for v in lay.n))]
