@@ -177,7 +177,9 @@ def read_plotfile(dirname: str) -> dict:
     cell_path = next(it)
     with open(os.path.join(dirname, cell_path + "_H")) as f:
         h = f.read().split("\n")
-    assert h[2].strip() == str(nc) and h[3].strip() == "0"
+    if h[2].strip() != str(nc) or h[3].strip() != "0":
+        raise ValueError(f"{dirname}: Cell_H holds {h[2].strip()} components with {h[3].strip()} "
+                         f"ghosts; the Header names {nc} components")
     nb = int(h[4].strip("( ").split()[0])
     box_list = [_box(h[5 + b]) for b in range(nb)]
     fabs = [ln.split() for ln in h if ln.startswith("FabOnDisk:")]
@@ -187,7 +189,8 @@ def read_plotfile(dirname: str) -> dict:
         with open(os.path.join(level_dir, fname), "rb") as f:
             f.seek(int(off))
             head = f.readline().decode()
-            assert head.startswith("FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))"), head
+            if not head.startswith("FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))"):
+                raise ValueError(f"{fname} at {off}: not a little-endian float64 FAB: {head[:80]!r}")
             n = [h_ - l_ + 1 for l_, h_ in zip(lo, hi)]
             vals = np.frombuffer(f.read(8 * nc * n[0] * n[1] * n[2]), dtype="<f8")
             data.append(vals.reshape(nc, n[2], n[1], n[0]).copy())
