@@ -8,8 +8,11 @@ hipGraph.  Box state is ghost-free ([5][e][e][e] float64, component outermost).
 ``fp32=True`` selects the FP32-flux mode (stencil arithmetic of hypterm and
 diffterm in FP32; state, RK3 accumulation, p/T recovery and dt in FP64).
 
-The tile shape of each box edge comes from a table the autotuner below
-measures (:func:`autotune`) and stores in ``tuning_gfx950.json``.
+The tile shape of each box edge comes from ``tuning_gfx950.json``: 32-wide tiles
+and z runs of min(edge, 16) for every edge, so that all boxes share one width class
+(measured on a mixed request stream; edges the table does not list get the same).
+The autotuner below (:func:`autotune`) times every shape per edge on batches of one
+edge, for workloads that are not mixed; its table can replace the file.
 """
 from __future__ import annotations
 

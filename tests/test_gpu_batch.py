@@ -113,6 +113,24 @@ def test_fp32_flux_error_after_100_steps(device):
     e32.close()
 
 
+def test_fp32_tile_widths_agree(device):
+    """The tile shape only decides which lane computes which cell, so the FP32 engine's
+    16-, 32- and 64-wide classes (and any z run) give the same bits as the default
+    32-wide tiles."""
+    edges = (16, 24, 40)
+    init = {e: compact_sinusoid(e, 70 + e, mach=0.2, kmax=3) for e in edges}
+    results = []
+    for tiles in ({}, {16: (16, 16), 24: (64, 7), 40: (16, 40)}, {16: (64, 4), 24: (16, 24), 40: (64, 9)}):
+        eng = BatchEngine(2 << 30, fp32=True, max_boxes=4, tuning=tiles)
+        ids = {e: eng.add(init[e].reshape(-1), e) for e in edges}
+        eng.step(5)
+        results.append({e: eng.read(ids[e]).cpu().numpy() for e in edges})
+        eng.close()
+    for r in results[1:]:
+        for e in edges:
+            assert np.array_equal(r[e], results[0][e]), f"edge {e}"
+
+
 def test_budget_admission(device):
     edge = 32
     need = box_bytes(edge, True)
