@@ -113,6 +113,30 @@ def test_fp32_flux_error_after_100_steps(device):
     e32.close()
 
 
+def test_fp32_flux_error_request_boxes_100_steps(device):
+    """The FP32-flux target on the served workload's own boxes: the first 8 requests of
+    the request stream (edges 24-128, Mach up to 0.3), 100 steps each, FP32-flux engine
+    against the FP64 engine (bit-identical to the oracle): per box and field, relative
+    L2 <= 1e-6."""
+    from cns_nospec_amd.batch import box_bytes
+    from cns_nospec_amd.serve import make_stream, request_state
+    reqs = make_stream(8, 0.0, 2024)
+    out = {}
+    for fp32 in (False, True):
+        eng = BatchEngine(sum(box_bytes(r.edge, fp32) for r in reqs) + (64 << 20), fp32=fp32, max_boxes=8)
+        ids = [eng.add(request_state(r).reshape(-1), r.edge) for r in reqs]
+        eng.step(100)
+        q = eng.query(ids)
+        assert all(q[i]["steps"] == 100 and q[i]["err"] == 0 for i in ids)
+        out[fp32] = [eng.read(i).cpu().numpy() for i in ids]
+        eng.close()
+    for r, a, b in zip(reqs, out[True], out[False]):
+        for l in range(NC):
+            rel_l2 = np.linalg.norm(a[l] - b[l]) / np.linalg.norm(b[l])
+            print(f"edge {r.edge} mach {r.mach:.3f} field {l}: rel L2 {rel_l2:.3e}")
+            assert rel_l2 <= 1e-6, f"edge {r.edge} field {l}: relative L2 {rel_l2:.3e}"
+
+
 def test_fp32_tile_widths_agree(device):
     """The tile shape only decides which lane computes which cell, so the FP32 engine's
     16-, 32- and 64-wide classes (and any z run) give the same bits as the default
