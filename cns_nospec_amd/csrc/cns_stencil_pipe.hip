@@ -951,14 +951,23 @@ hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, 
 }
 
 
+// z runs of the column kernels (ctoprim + wz, wz): long runs keep the 8-plane window
+// warm-up small; small boxes shorten them, down to 8 planes, until a launch has 512
+// workgroups (at 64^3 the ctoprim + wz pass had 46 workgroups of 64-plane runs).
+static int column_kz(int kz, int64_t col_blocks, int nz, int nbox)
+{
+    while (kz > 8 && col_blocks * nbox * ((nz + kz - 1) / kz) < 512) kz /= 2;
+    return kz;
+}
+
 hipError_t launch_ctoprim_wz(const Geom &g, const double *u, double *q, double *wz,
                              unsigned long long *acc, double dx0, double dx1, double dx2, double dxi2,
                              bool full, hipStream_t s, const BoxSet &bs)
 {
-    const int kz = 64;
     if (bs.n < 1 || bs.n > 65535) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)((g.pg + kCtoBlock - 1) / kCtoBlock), (unsigned)((g.g2 + kz - 1) / kz),
-                    (unsigned)bs.n);
+    const int64_t cols = (g.pg + kCtoBlock - 1) / kCtoBlock;
+    const int kz = column_kz(64, cols, g.g2, bs.n);
+    const dim3 grid((unsigned)cols, (unsigned)((g.g2 + kz - 1) / kz), (unsigned)bs.n);
 #define CTO_WZ(C, F) \
     ctoprim_wz_kernel<C, F><<<grid, kCtoBlock, 0, s>>>(g, u, q, wz, acc, dx0, dx1, dx2, dxi2, kz, bs)
     if (acc)
@@ -971,9 +980,10 @@ hipError_t launch_ctoprim_wz(const Geom &g, const double *u, double *q, double *
 
 hipError_t launch_wz(const Geom &g, const double *q, double *wz, double dxi2, hipStream_t s, const BoxSet &bs)
 {
-    const int kz = g.n2 > 128 ? 128 : g.n2;
     if (bs.n < 1 || bs.n > 65535) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)((g.pg + 255) / 256), (unsigned)((g.n2 + kz - 1) / kz), (unsigned)bs.n);
+    const int64_t cols = (g.pg + 255) / 256;
+    const int kz = column_kz(g.n2 > 128 ? 128 : g.n2, cols, g.n2, bs.n);
+    const dim3 grid((unsigned)cols, (unsigned)((g.n2 + kz - 1) / kz), (unsigned)bs.n);
     wz_kernel<<<grid, 256, 0, s>>>(g, q + QW * g.cg, wz, dxi2, kz, bs);
     return hipGetLastError();
 }
