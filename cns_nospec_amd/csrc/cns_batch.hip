@@ -33,7 +33,10 @@ constexpr int kLdsCells = 864;  // max (TY+8)*(TX+8) over TX in {16, 32, 64}
 
 // Global array indexed by an unsigned 32-bit cell offset, turned into a 32-bit
 // byte offset from a uniform base, so that each access is one scalar-base
-// global_load/store with a 32-bit lane offset (a box component is < 4 GiB).
+// global_load/store with a 32-bit lane offset.  An index must stay inside ONE
+// component (< e^3 <= 2^27 cells, so index * 8 < 2^30): the component base is
+// formed by operator+, in 64-bit pointer arithmetic, because a box's five
+// components together pass 4 GiB for edges >= 476.
 template <typename T>
 struct GArr {
     T GAS *p;
@@ -41,10 +44,7 @@ struct GArr {
     {
         return *(T GAS *)((char GAS *)p + i * (unsigned)sizeof(T));
     }
-    __device__ __forceinline__ GArr operator+(unsigned n) const
-    {
-        return GArr{(T GAS *)((char GAS *)p + n * (unsigned)sizeof(T))};
-    }
+    __device__ __forceinline__ GArr operator+(size_t n) const { return GArr{p + n}; }
 };
 template <typename T>
 __device__ __forceinline__ GArr<T> garr(void *p)
@@ -433,8 +433,16 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
     const GArr<double> c_r = garr<double>((void *)cur), c_mx = c_r + e3, c_my = c_r + 2 * e3,
                        c_mz = c_r + 3 * e3, c_E = c_r + 4 * e3;
     const GArr<double> b_0 = garr<double>(B.U), o_0 = garr<double>(out);
+    GArr<double> b_c[NC], o_c[NC];
+#pragma unroll
+    for (int l = 0; l < NC; l++) {
+        b_c[l] = b_0 + (size_t)l * e3;
+        o_c[l] = o_0 + (size_t)l * e3;
+    }
     const GArr<FT> qi = garr<FT>(B.Q[stage - 1]), qo = garr<FT>(B.Q[stage % 3]);
+    const GArr<FT> qo_v = qo + e3, qo_w = qo + 2 * e3, qo_p = qo + 3 * e3, qo_t = qo + 4 * e3;
     const GArr<FT> din = garr<FT>(B.D);
+    const GArr<FT> d_c[4] = {din, din + e3, din + 2 * e3, din + 3 * e3};
     const GArr<FT> q_u = qi, q_v = qi + e3, q_w = qi + 2 * e3, q_p = qi + 3 * e3;
     const double dt = B.dt;
     const double ra = stage == 1 ? 0.0E0 : (stage == 2 ? ThreeQuarters : OneThird);
@@ -510,10 +518,10 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
         if (t.active) {  // this cell's RK operands
             rho_c = c_r[zo];
 #pragma unroll
-            for (int l = 0; l < 4; l++) rkd[l] = (double)din[l * e3 + zo];
+            for (int l = 0; l < 4; l++) rkd[l] = (double)d_c[l][zo];
             if (stage != 1) {
 #pragma unroll
-                for (int l = 0; l < NC; l++) rkb[l] = b_0[l * e3 + zo];
+                for (int l = 0; l < NC; l++) rkb[l] = b_c[l][zo];
             }
         }
         if (t.active) {
@@ -568,15 +576,15 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
                 const double df = dl + (double)F[l];
                 const double x = (l == 0 ? rho_c : cur_c[l]) + dt * df;
                 o[l] = stage == 1 ? x : ra * rkb[l] + rb * x;
-                o_0[l * e3 + zo] = o[l];
+                o_c[l][zo] = o[l];
             }
             double u, v, w, p, T;
             prim_cell(o[0], o[1], o[2], o[3], o[4], u, v, w, p, T);
             qo[zo] = (FT)u;
-            qo[zo + e3] = (FT)v;
-            qo[zo + 2 * e3] = (FT)w;
-            qo[zo + 3 * e3] = (FT)(p - pref);
-            qo[zo + 4 * e3] = (FT)T;
+            qo_v[zo] = (FT)v;
+            qo_w[zo] = (FT)w;
+            qo_p[zo] = (FT)(p - pref);
+            qo_t[zo] = (FT)T;
             if (stage == 3) {
                 const unsigned long long cb = cour_bits(o[0], u, v, w, p, dxd);
                 cmax = cmax > cb ? cmax : cb;

@@ -952,11 +952,12 @@ hipError_t launch_hypterm_cons(const Geom &g, const double *cons, double *flux, 
 
 
 // z runs of the column kernels (ctoprim + wz, wz): long runs keep the 8-plane window
-// warm-up small; small boxes shorten them, down to 8 planes, until a launch has 512
-// workgroups (at 64^3 the ctoprim + wz pass had 46 workgroups of 64-plane runs).
+// warm-up small; small boxes shorten them, down to 8 planes (or the starting run when
+// that is shorter), until a launch has 512 workgroups (at 64^3 the ctoprim + wz pass
+// had 46 workgroups of 64-plane runs).
 static int column_kz(int kz, int64_t col_blocks, int nz, int nbox)
 {
-    while (kz > 8 && col_blocks * nbox * ((nz + kz - 1) / kz) < 512) kz /= 2;
+    while (kz > 8 && col_blocks * nbox * ((nz + kz - 1) / kz) < 512) kz = kz / 2 > 8 ? kz / 2 : 8;
     return kz;
 }
 
