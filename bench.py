@@ -8,18 +8,22 @@ ghost fill, ctoprim + Courant reduction, diffterm, hypterm, RK update, three
 stages, dt synchronised to the host every step as in the reference) on one
 periodic 512^3 box (520^3 ghosted, ng = 4), seeded sinusoidal state with
 random phases on wavenumbers 1-4.  A step is one RK3 step of the whole box;
-value = interior cells x steps / wall time of the timed steps (inputs resident
-in HBM).  The other workloads (64^3 plain, batched, served) are reported
-under ``secondary``.
+five windows of exactly ``--steps`` steps are timed and the median window's
+rate is reported: value = interior cells x steps / wall time of that window
+(inputs resident in HBM).  The other workloads (the cell64 CGS deck, the
+64-box batch, the 2000-request served stream) are reported under
+``secondary``.
 
 ``roofline``: the dominant kernel's algorithmic HBM bytes per launch (each
-ABI array element read once over its needed extent, each output written once)
-divided by its average duration, measured with HIP events around every launch
-inside the timed region on the launch stream.  ``traffic`` is the measured HBM
-bytes per launch from rocprofv3 PMC counters (profiles/hbm_traffic.txt) when
-present.  ``cpu_baseline``: the reference cpu/ path itself (oracle/_ref,
-compiled from the reference sources) timed on one host core on a bounded
-sample, or the oracle restatement where that build is absent.
+array element it needs read once over its needed extent, each output written
+once) divided by its average duration, measured with HIP events around every
+launch inside the timed windows on the launch stream.  ``traffic`` is the
+measured HBM bytes per launch from rocprofv3 PMC counters
+(profiles/hbm_traffic.txt).  ``step_bytes`` sets the step's measured traffic
+beside the reference ABI's bytes and the fused composition's own minimum.
+``cpu_baseline``: the reference cpu/ path itself (oracle/_ref, compiled from
+the reference sources) timed on one host core at 32^3, 64^3 and 128^3, or the
+oracle restatement where that build is absent.
 """
 from __future__ import annotations
 
@@ -40,8 +44,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
 
 
 def algorithmic_bytes(kind: str, n: int, stage: int = 2) -> float:
-    """Per-launch algorithmic HBM bytes of one ABI kernel on an n^3 box (f64):
-    each input element read once over its needed extent, each output written once."""
+    """Per-launch algorithmic HBM bytes of one kernel on an n^3 box (f64): each input
+    element read once over its needed extent, each output written once."""
     g3, n3 = (n + 8) ** 3, n ** 3
     rk = 20 * n3 if stage == 1 else 25 * n3
     table = {
@@ -50,19 +54,56 @@ def algorithmic_bytes(kind: str, n: int, stage: int = 2) -> float:
         "diffterm": 4 * g3 + 5 * n3,
         "hypterm": 8 * g3 + 5 * n3,
         "rk": rk,
-        # the fused pass, averaged over a step's three stages: cons and q over their halos,
-        # the cell's rho and D without its zero rho row (5c), the stage base in stages 2-3
-        # (10c / 3), the result (5c), F in stage 3 only (5c / 3), and the primitives of
-        # the result stored for the next stage (u, v, w, T after stage 1, all six after
-        # stage 2: 10c / 3)
-        "hypterm_rk": 8 * g3 + 15 * n3 + 10 * n3 / 3,
+        # the fused hypterm + RK pass, averaged over a step's three stages: the five
+        # conserved fields over their halos (5g; u, v, w, p are derived from them in
+        # the tile), D without its zero rho row (4c), the stage base in stages 2-3
+        # (10c / 3), the result (5c), F in stage 3 only (5c / 3), and the primitives
+        # of the result stored for the next stage (u, v, w, T after stage 1, all six
+        # after stage 2: 10c / 3)
+        "hypterm_rk": 5 * g3 + 9 * n3 + 25 * n3 / 3,
         "fill_ctoprim": 10 * (g3 - n3) + 11 * g3,
     }
     return 8.0 * table[kind]
 
 
 def step_algorithmic_bytes(n: int) -> float:
+    """The reference's ABI traffic per step: every array of the unfused composition
+    read and written once per use."""
     return 8.0 * (99 * (n + 8) ** 3 + 70 * n ** 3)
+
+
+def fused_step_min_bytes(n: int) -> float:
+    """What the fused composition itself must move per step (f64): fills 50(g-c)
+    (state shell in three stages, plus u, v, w, T and then all six primitives in
+    stages 2 and 3), ctoprim + wz 10g (stage 1), wz 2g (stages 2 and 3), diffterm
+    5g + 4c/4c/5c, fused hypterm + RK 3 x (5g + 17.33c): 94g + 15c in all."""
+    g3, n3 = (n + 8) ** 3, n ** 3
+    return 8.0 * (94 * g3 + 15 * n3)
+
+
+# Minimum of the batched / fused path per interior cell-step: read U, write U1; read
+# U1 and U, write U2; read U2 and U, write U (40 doubles).
+BATCHED_MIN_BYTES_PER_CELL_STEP = 320.0
+
+
+WINDOWS = 5  # timed windows of the headline; the median is reported
+
+
+def step_traffic(kern: dict, edge: int):
+    """Measured HBM bytes of one step: per kernel kind, the calibrated bytes per launch
+    (profiles/hbm_traffic.txt) times the launches per step; None when a kind has no
+    measurement at this edge."""
+    steps_kinds = {k: v for k, v in kern.items() if v[1] > 0}
+    if not steps_kinds:
+        return None
+    launches_per_step = {"fill": 3, "ctoprim": 1, "diffterm": 3, "hypterm_rk": 3, "wz": 2}
+    total = 0.0
+    for kind, per_step in launches_per_step.items():
+        b = read_traffic(kind, edge)
+        if b is None:
+            return None
+        total += b * per_step
+    return total
 
 
 def read_traffic(kind: str, edge: int):
@@ -89,7 +130,10 @@ CHUNK = 20
 
 
 def run_plain(edge: int, steps: int, warmup: int, seed: int = 2024, kmax: int = 4,
-              profile: bool = True):
+              profile: bool = True, windows: int = 1):
+    """`windows` timed windows of exactly `steps` steps each (after `warmup` untimed
+    steps); the rate reported is the median window's.  Kernel times are HIP events
+    around every launch of every window."""
     import torch
     from cns_nospec_amd import ops
     from cns_nospec_amd.layout import BoxLayout
@@ -109,42 +153,94 @@ def run_plain(edge: int, steps: int, warmup: int, seed: int = 2024, kmax: int = 
         st.new_advance(U, Unew, Q, D, F)
     if profile:
         st.profile(True)
-    el, dts = 0.0, []
-    while len(dts) < steps:
-        n = min(CHUNK, steps - len(dts))
-        seed_state()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        dts += [st.new_advance(U, Unew, Q, D, F) for _ in range(n)]
-        torch.cuda.synchronize()
-        el += time.perf_counter() - t0
+    window_s, dts = [], []
+    for _ in range(windows):
+        el, done = 0.0, 0
+        while done < steps:
+            n = min(CHUNK, steps - done)
+            seed_state()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dts += [st.new_advance(U, Unew, Q, D, F) for _ in range(n)]
+            torch.cuda.synchronize()
+            el += time.perf_counter() - t0
+            done += n
+        window_s.append(el)
     kern = st.profile_read() if profile else {}
     st.profile(False)
     st.close()
     del U, Unew, Q, D, F
     torch.cuda.empty_cache()
+    el = sorted(window_s)[len(window_s) // 2]
     return {"edge": edge, "elapsed_s": el, "cell_steps_per_s": edge ** 3 * steps / el,
-            "ms_per_step": 1e3 * el / steps, "dt_last": dts[-1], "kernels": kern}
+            "ms_per_step": 1e3 * el / steps, "dt_last": dts[-1], "kernels": kern,
+            "window_ms_per_step": [round(1e3 * w / steps, 3) for w in window_s]}
+
+
+def run_cgs_deck(edge: int = 64, steps: int = 100, warmup: int = 5):
+    """The reference's cell64 deck (fortran90/cell64: one 64^3 box, 100 steps, CGS
+    initial condition of fortran90/init_data.f90:43-85) through cns_advance, the
+    configuration of the reference's published C2050 figure (cns_nospec.xls, 64^3 x 100
+    steps in 2.7245 s)."""
+    import torch
+    from cns_nospec_amd import ops
+    from cns_nospec_amd.layout import BoxLayout
+    from cns_nospec_amd.state import cgs_state
+
+    lay = BoxLayout.cube(edge)
+    U0 = torch.from_numpy(cgs_state(lay)).to("cuda")
+    U, Unew, Q, D, F = ops.allocate_variables(lay, "cuda")
+    st = ops.Stepper(lay)
+    U.copy_(U0)
+    for _ in range(warmup):
+        st.new_advance(U, Unew, Q, D, F)
+    U.copy_(U0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.new_advance(U, Unew, Q, D, F)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st.close()
+    rate = edge ** 3 * steps / el
+    return {"workload": f"cell{edge} deck: one periodic {edge}^3 box, CGS initial condition, "
+                        f"{steps} RK3 steps through cns_advance, dt to the host every step",
+            "value": rate, "unit": "cell-steps/s", "ms_per_step": 1e3 * el / steps,
+            "steps": steps, "vs_c2050_published": rate / C2050_64_CELL_STEPS}
 
 
 @contextlib.contextmanager
 def quiet_stdout():
     """The reference prints `dt, courno` each step; keep it off our one JSON line."""
+    import ctypes
+    libc = ctypes.CDLL(None)
     sys.stdout.flush()
+    libc.fflush(None)
     saved = os.dup(1)
     null = os.open(os.devnull, os.O_WRONLY)
     os.dup2(null, 1)
     try:
         yield
     finally:
+        libc.fflush(None)  # C stdio's buffered lines go to /dev/null, not to our stdout
         os.dup2(saved, 1)
         os.close(null)
         os.close(saved)
 
 
-def cpu_baseline(edge: int = 128, budget_s: float = 15.0):
-    """Reference cpu/ new_advance on one core, as many steps of an edge^3 box as fit
-    the time budget (at least 2)."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_one(edge: int, budget_s: float):
+    """Reference cpu/ new_advance on one core: as many steps of an edge^3 box as fit
+    the time budget (at least 2).  Returns (cell-steps/s, steps, seconds, kind)."""
     import numpy as np
     import oracle
     from cns_nospec_amd.layout import NC, NQ, BoxLayout
@@ -163,6 +259,7 @@ def cpu_baseline(edge: int = 128, budget_s: float = 15.0):
     else:
         kind = "port"
         orc = oracle.Oracle()
+        orc.set_threads(1)
         Unew, Q = lay.new_ghosted(NC), lay.new_ghosted(NQ)
         D, F = lay.new_interior(NC), lay.new_interior(NC)
         dx = box_dx(edge)
@@ -174,9 +271,25 @@ def cpu_baseline(edge: int = 128, budget_s: float = 15.0):
             step()
             n += 1
         el = time.perf_counter() - t0
-    return {"value": edge ** 3 * n / el, "unit": "cell-steps/s", "cores": 1, "kind": kind,
-            "sample": f"{n} RK3 steps of one {edge}^3 box (seeded sinusoids), single thread, "
-                      f"{el:.1f} s; the path has no threading (its OpenMP is commented out)"}
+    return edge ** 3 * n / el, n, el, kind
+
+
+def cpu_baseline(budgets=((32, 4.0), (64, 6.0), (128, 12.0))):
+    """The reference cpu/ path (oracle/_ref: the unmodified sources, g++ -O2 as
+    cpu/Makefile:2) on one host core at 32^3, 64^3 and 128^3 (cpu/main.cpp:27-34 runs
+    whatever box its input names); the value is the 128^3 rate, the size closest to
+    the 512^3 GPU workload that a bounded CPU sample can step (one 512^3 step takes
+    about a minute on one core)."""
+    by_edge, parts, kind = {}, [], "port"
+    for edge, budget in budgets:
+        rate, n, el, kind = cpu_baseline_one(edge, budget)
+        by_edge[str(edge)] = rate
+        parts.append(f"{n} steps of {edge}^3 in {el:.1f} s")
+    return {"value": by_edge[str(budgets[-1][0])], "unit": "cell-steps/s", "cores": 1, "kind": kind,
+            "cpu_model": cpu_model(), "by_edge": by_edge,
+            "sample": "RK3 steps of one box (seeded sinusoids on wavenumbers 1-4), single thread "
+                      "(the path has no threading: its OpenMP is commented out): " + "; ".join(parts)
+                      + f"; value = the {budgets[-1][0]}^3 rate"}
 
 
 def cpu_baseline_all_cores(edge: int = 128, budget_s: float = 10.0):
@@ -203,22 +316,8 @@ def cpu_baseline_all_cores(edge: int = 128, budget_s: float = 10.0):
         n += 1
     el = time.perf_counter() - t0
     return {"value": edge ** 3 * n / el, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} RK3 steps of one {edge}^3 box (seeded sinusoids), {threads} OpenMP "
+            "cpu_model": cpu_model(), "sample": f"{n} RK3 steps of one {edge}^3 box (seeded sinusoids), {threads} OpenMP "
                       f"threads, {el:.1f} s"}
-
-
-BATCH_EDGES = (16, 24, 32, 48, 64, 96)
-
-
-def batched_requests(nbox: int = 64, steps: int = 50, seed: int = 4):
-    """64 boxes, edges uniform over {16,...,96}, Mach U[0.05, 0.3], own seeded states."""
-    import numpy as np
-    from cns_nospec_amd.serve import Request
-    rng = np.random.default_rng(seed)
-    edges = rng.choice(BATCH_EDGES, nbox)
-    machs = rng.uniform(0.05, 0.3, nbox)
-    seeds = rng.integers(0, 2 ** 31 - 1, nbox)
-    return [Request(i, 0.0, int(edges[i]), steps, int(seeds[i]), float(machs[i])) for i in range(nbox)]
 
 
 def field_errors(a, b):
@@ -287,7 +386,8 @@ def run_plain_boxes(reqs):
 
 
 def batched_secondary():
-    reqs = batched_requests()
+    from cns_nospec_amd.serve import batch_mix
+    reqs = batch_mix()
     plain, s64 = run_plain_boxes(reqs)
     b64, sb64 = run_batched(reqs, False)
     b32, sb32 = run_batched(reqs, True)
@@ -319,44 +419,56 @@ def fp32_error_secondary(nbox: int = 8, steps: int = 100, seed: int = 2024):
             "per_field_worst": worst_errors([field_errors(a, b) for a, b in zip(s32, s64)])}
 
 
-def served_secondary(nreq: int, seed: int = 2024):
+def served_secondary(nreq: int, seed: int = 2024, poisson_nreq: int = 200):
     """The mixed-size request stream (edge 24-128, 50-500 steps) served twice: as a
-    burst (every request waiting at t = 0: both servers saturated, so the aggregate
-    rates compare capacities), and as a Poisson stream offering 1.2x the plain path's
-    measured capacity (latency under load)."""
-    import numpy as np
+    burst of all `nreq` requests (every request waiting at t = 0: both servers
+    saturated, so the aggregate rates compare capacities), and as a Poisson stream of
+    the first `poisson_nreq` requests offering 1.2x the plain path's measured capacity
+    (latency under load).  Progress goes to stderr."""
     import torch
     from cns_nospec_amd.serve import BatchServer, PlainServer, make_stream
     budget = 200 << 30
     keep = tuple(range(min(4, nreq)))
+    every = max(1, nreq // 10)
     burst = make_stream(nreq, 0.0, seed)
     work = sum(r.edge ** 3 * r.steps for r in burst)
-    res_p, plain = PlainServer(keep_states=keep).run(burst)
+    res_p, plain = PlainServer(keep_states=keep, progress_every=every).run(burst)
     torch.cuda.empty_cache()
-    srv = BatchServer(budget, fp32=True, keep_states=keep)
+    srv = BatchServer(budget, fp32=True, keep_states=keep, progress_every=every)
     res_b, bat = srv.run(burst)
     srv.close()
     torch.cuda.empty_cache()
     errs = worst_errors([field_errors(res_b[i].state.cpu().numpy(), res_p[i].state.cpu().numpy())
                          for i in keep])
+    npo = min(nreq, poisson_nreq)
     lam = 1.2 * plain["cell_steps_per_s"] / (work / nreq)
-    stream = make_stream(nreq, lam, seed)
+    stream = make_stream(npo, lam, seed)
     _, plain_l = PlainServer().run(stream)
     torch.cuda.empty_cache()
     srv = BatchServer(budget, fp32=True)
     _, bat_l = srv.run(stream)
     srv.close()
     torch.cuda.empty_cache()
+    ach = BATCHED_MIN_BYTES_PER_CELL_STEP * bat["cell_steps_per_s"] / 1e9
+    measured = read_traffic("batched_fp32_step", 0)
     return {"workload": f"{nreq} requests (seed {seed}), edge U{{24..128}}, steps U{{50..500}}, "
                         "Mach U[0.05,0.3], 200 GB budget, FP32-flux continuous batching vs "
-                        "plain FP64 one box at a time",
+                        "plain FP64 one box at a time (cns_advance, dt to the host each step)",
             "cell_steps": int(work), "unit": "cell-steps/s",
-            "burst": {"plain_fp64": plain, "batched_fp32_flux": bat,
+            "burst": {"requests": nreq, "plain_fp64": plain, "batched_fp32_flux": bat,
                       "ratio": bat["cell_steps_per_s"] / plain["cell_steps_per_s"]},
-            "poisson": {"rate_per_s": lam, "offered_vs_plain_capacity": 1.2,
+            "poisson": {"requests": npo,
+                        "workload": f"the first {npo} requests of the same seeded stream, Poisson "
+                                    f"arrivals at 1.2x the plain path's burst capacity",
+                        "rate_per_s": lam, "offered_vs_plain_capacity": 1.2,
                         "plain_fp64": plain_l, "batched_fp32_flux": bat_l},
+            "roofline": {"bound": "hbm", "basis": "320 B per interior cell-step (read U, write U1; "
+                                                  "read U1 and U, write U2; read U2 and U, write U)",
+                         "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "traffic_bytes_per_cell_step": measured},
             "fp32_flux_error_vs_fp64": errs,
-            "error_sample": f"requests {list(keep)} of the burst run, final state"}
+            "error_sample": f"requests {list(keep)} of the burst run, final state (50-500 steps)"}
 
 
 def domain_secondary(edge: int = 256, m: int = 64, steps: int = 10, warmup: int = 2):
@@ -461,12 +573,15 @@ def main():
     ap.add_argument("--edge", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=200,
-                    help="requests of the served workload (0 skips it)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests of the served burst (0 skips it); the Poisson leg serves "
+                         "the first 200 of them")
     ap.add_argument("--kernels-once", action="store_true",
                     help="run each step kernel a few times and a calibration copy (for rocprofv3)")
     ap.add_argument("--batched-once", action="store_true",
                     help="a few steps of the batched engines on the request mix (for rocprofv3)")
+    ap.add_argument("--once-requests", type=int, default=200,
+                    help="boxes of the request mix resident in --batched-once")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -485,10 +600,10 @@ def main():
         kernels_once(args.edge)
         return
     if args.batched_once:
-        batched_once(args.serve_requests or 200)
+        batched_once(args.once_requests)
         return
 
-    main_run = run_plain(args.edge, args.steps, args.warmup)
+    main_run = run_plain(args.edge, args.steps, args.warmup, windows=WINDOWS)
     kern = main_run["kernels"]
     dom = max(kern, key=lambda k: kern[k][0]) if kern else None
     roof = None
@@ -497,11 +612,14 @@ def main():
         avg_s = ms_tot / cnt / 1e3
         nbytes = algorithmic_bytes(dom, args.edge)
         ach = nbytes / avg_s / 1e9
+        traffic = read_traffic(dom, args.edge)
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": read_traffic(dom, args.edge),
-                "algorithmic_bytes": nbytes, "avg_ms": round(avg_s * 1e3, 4)}
+                "traffic": traffic, "algorithmic_bytes": nbytes, "avg_ms": round(avg_s * 1e3, 4),
+                "traffic_vs_algorithmic": round(traffic / nbytes, 3) if traffic else None}
     step_bytes = step_algorithmic_bytes(args.edge)
+    fused_min = fused_step_min_bytes(args.edge)
+    measured_step = step_traffic(kern, args.edge)
     out = {
         "metric": METRIC,
         "value": main_run["cell_steps_per_s"],
@@ -519,17 +637,22 @@ def main():
                                f"{args.edge}^3 box ({args.edge + 8}^3 ghosted, ng=4), seeded "
                                f"sinusoids on wavenumbers 1-4, dt to host every step",
                    "edge": args.edge, "ng": 4, "cfl": 0.5, "eta": 1.8e-4, "alam": 150.0},
+        "timing": f"median of {WINDOWS} windows of exactly {args.steps} steps each (window "
+                  f"ms/step {main_run['window_ms_per_step']})",
         "step_hbm_fraction": round(step_bytes / (main_run["ms_per_step"] / 1e3) / 1e9
                                    / HBM_PEAK_GBS, 4),
+        "step_bytes": {"abi_unfused": step_bytes, "fused_minimum": fused_min,
+                       "measured": measured_step,
+                       "measured_vs_fused_minimum": round(measured_step / fused_min, 3)
+                       if measured_step else None,
+                       "fused_minimum_hbm_fraction": round(fused_min / (main_run["ms_per_step"] / 1e3)
+                                                           / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": roof,
         "kernels_ms_per_launch": {k: round(v[0] / v[1], 4) for k, v in kern.items()},
     }
     if not args.no_secondary:
         sec = {}
-        r64 = run_plain(64, 100, 5, seed=64, kmax=2, profile=False)
-        sec["plain_64"] = {"value": r64["cell_steps_per_s"], "unit": "cell-steps/s",
-                           "ms_per_step": r64["ms_per_step"], "steps": 100,
-                           "vs_c2050_published": r64["cell_steps_per_s"] / C2050_64_CELL_STEPS}
+        sec["cgs_cell64"] = run_cgs_deck(64, 100, 5)
         sec["batched"] = batched_secondary()
         sec["fp32_flux_error_100_steps"] = fp32_error_secondary()
         sec["domain"] = domain_secondary()

@@ -16,6 +16,7 @@ alone).
 from __future__ import annotations
 
 import math
+import sys
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -66,6 +67,21 @@ def make_stream(n: int, rate: float, seed: int, edge_range=(24, 128), steps_rang
             for i in range(n)]
 
 
+BATCH_MIX_EDGES = (16, 24, 32, 48, 64, 96)
+
+
+def batch_mix(nbox: int = 64, steps: int = 50, seed: int = 4, edges=BATCH_MIX_EDGES,
+              mach_range=(0.05, 0.3)):
+    """The batched workload: nbox independent boxes, edges uniform over `edges`,
+    Mach ~ U(mach_range), each with its own seeded sinusoidal state, all advanced
+    `steps` RK3 steps (64 boxes of {16,...,96} x 50 steps by default)."""
+    rng = np.random.default_rng(seed)
+    e = rng.choice(edges, nbox)
+    machs = rng.uniform(mach_range[0], mach_range[1], nbox)
+    seeds = rng.integers(0, 2 ** 31 - 1, nbox)
+    return [Request(i, 0.0, int(e[i]), steps, int(seeds[i]), float(machs[i])) for i in range(nbox)]
+
+
 def request_state(req: Request, device="cuda"):
     """The request's initial ghost-free U (5, e, e, e) on the device.
 
@@ -111,9 +127,10 @@ def summarize(results, wall: float):
 
 
 def _progress(every: int, done: int, total: int, t: float):
-    """One line per `every` finished requests (long runs stay visibly alive)."""
+    """One line on stderr per `every` finished requests (long runs stay visibly alive
+    without touching stdout)."""
     if every and done % every == 0:
-        print(f"served {done}/{total} requests at t={t:.1f} s", flush=True)
+        print(f"served {done}/{total} requests at t={t:.1f} s", file=sys.stderr, flush=True)
 
 
 class _Clock:

@@ -18,6 +18,12 @@ Fixtures (float64, interior cells, [l][k][j][i] order, AoS-independent):
   sample cells; plus U0.
 * ``cgs_16.npz``       -- the reference's own CGS initial condition
   (fortran90/init_data.f90) at 16^3, U after 3 steps and the dts.
+* ``cgs_cell32.npz``, ``cgs_cell64.npz`` -- the reference's own cell32 / cell64 input
+  decks (fortran90/cell32, fortran90/cell64: n_cell 32 / 64, one box, nsteps 100,
+  cfl 0.5, eta 1.8e-4, alam 1.5e2) on the CGS initial condition: per step dt, SHA-256
+  of the interior bytes, per-field min/max/sum/sum of squares and 256 seeded sample
+  cells; plus the SHA-256 of U0 (so a change of the generator is caught).
+  ``python tests/golden/make_golden.py cgs100`` rewrites only these two.
 * ``refio_16.npz``     -- the reference PROGRAM (oracle/_ref/cpu, cpu/main.cpp) run on
   testcase files holding the U0 of steps_16 and of cgs_16 (ghosts zero): SHA-256 and
   size of its ``output`` file after 3 steps and its per-step "dt, courno" lines.
@@ -127,10 +133,46 @@ def refio():
     np.savez_compressed(os.path.join(HERE, "refio_16.npz"), **out)
 
 
+CGS_DECKS = (("cgs_cell32.npz", 32), ("cgs_cell64.npz", 64))  # fortran90/cell32, cell64
+CGS_STEPS = 100
+
+
+def cgs100():
+    """The cell32 / cell64 decks through the reference cpu/ new_advance, streamed step
+    by step (only digests are kept)."""
+    for fname, n in CGS_DECKS:
+        lay = BoxLayout.cube(n)
+        U = cgs_state(lay)
+        ref = oracle.Reference()
+        ref.setup(lay.n, box_dx(lay.n), CFL, ETA, ALAM)
+        Ua = lay.ghosted_to_aos(U, NC)
+        Unew = np.zeros_like(Ua)
+        Q = np.zeros(lay.dimg + (NQ,))
+        D = np.zeros(lay.n + (NC,))
+        F = np.zeros(lay.n + (NC,))
+        rng = np.random.default_rng(4242 + n)
+        idx = np.stack([rng.integers(0, NC, 256)] + [rng.integers(0, n, 256) for _ in range(3)], axis=1)
+        dts, shas, stats, samples = [], [], [], []
+        for _ in range(CGS_STEPS):
+            dts.append(ref.new_advance(Ua, Unew, Q, D, F))
+            cur = interior_aos(lay, Ua, NC)
+            shas.append(hashlib.sha256(cur.tobytes()).hexdigest())
+            stats.append(stats_of(cur))
+            samples.append(cur[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]])
+        u0 = interior_soa(lay, U, NC)
+        np.savez_compressed(os.path.join(HERE, fname), n=n, steps=CGS_STEPS, dt=np.array(dts),
+                            sha256=np.array(shas), stats=np.stack(stats), sample_idx=idx,
+                            samples=np.stack(samples),
+                            u0_sha256=np.array(hashlib.sha256(u0.tobytes()).hexdigest()))
+
+
 def main():
     os.makedirs(HERE, exist_ok=True)
     if sys.argv[1:] == ["refio"]:
         refio()
+        return
+    if sys.argv[1:] == ["cgs100"]:
+        cgs100()
         return
     lay16 = BoxLayout.cube(16)
     np.savez_compressed(os.path.join(HERE, "kernels_16.npz"), **kernels(lay16, SEED_16))
@@ -160,6 +202,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "cgs_16.npz"), n=16, U0=interior_soa(lay16, U, NC),
                         dt=dts, U3=kept[3])
     refio()
+    cgs100()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

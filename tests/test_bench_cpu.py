@@ -5,8 +5,11 @@ step's HBM fraction, and the error summaries.
   cell-step at 32^3, 1688 at 64^3, 1389.7 at 512^3 (the per-kernel terms summed over a
   step's three stages must give the same total).
 * The fused hypterm + RK kernel's per-launch figure is the mean of its three stages:
-  8g + 14c (stage 1: cons over halos, rho and D, result, u v w T), 8g + 21c (stage 2:
-  plus the base, all six primitives), 8g + 20c (stage 3: the base, the result and F).
+  5g + 13c (stage 1: the five conserved fields over halos, D without its rho row, the
+  result, u v w T), 5g + 20c (stage 2: plus the base, all six primitives), 5g + 19c
+  (stage 3: the base, the result and F).
+* The fused step's own minimum is the sum of its launches: 94g + 15c doubles, 121.8 GB
+  at 512^3.
 * profiles/hbm_traffic.txt parses, and the measured bytes of the fused kernel exceed its
   algorithmic bytes (halo re-reads), as a measurement of real traffic must."""
 import numpy as np
@@ -31,8 +34,23 @@ def test_kernel_terms_sum_to_the_step(n):
 def test_fused_kernel_bytes_are_the_stage_mean():
     n = 512
     g, c = (n + 8) ** 3, n ** 3
-    stages = [8 * g + 14 * c, 8 * g + 21 * c, 8 * g + 20 * c]
+    stages = [5 * g + 13 * c, 5 * g + 20 * c, 5 * g + 19 * c]
     assert bench.algorithmic_bytes("hypterm_rk", n) == pytest.approx(8 * sum(stages) / 3, rel=1e-12)
+
+
+@pytest.mark.parametrize("n", [32, 512])
+def test_fused_step_minimum_is_the_sum_of_its_launches(n):
+    g, c = (n + 8) ** 3, n ** 3
+    shell = g - c
+    fills = 10 * shell + 2 * (5 + 4) * shell + 2 * (5 + 6) * shell  # U; T + uvwT; Unew + Q
+    ctoprim_wz = 5 * g + 4 * g + g  # U in, u v w T and wz out
+    wz = 2 * (g + g)  # w in, wz out, stages 2 and 3
+    diff = 3 * (4 * g + g) + (4 + 4 + 5) * c  # u v w T and wz in; D out (rho row in stage 3)
+    hyp = (5 * g + 13 * c) + (5 * g + 20 * c) + (5 * g + 19 * c)
+    total = 8 * (fills + ctoprim_wz + wz + diff + hyp)
+    assert bench.fused_step_min_bytes(n) == pytest.approx(total, rel=1e-12)
+    if n == 512:
+        assert total / 1e9 == pytest.approx(121.8, abs=0.05)
 
 
 def test_measured_traffic_file():

@@ -135,6 +135,42 @@ def test_cgs_16_matches_reference(device):
     assert np.array_equal(lay.interior_of(lay.gview(host(out[-1][1]), NC)), g["U3"])
 
 
+@pytest.mark.parametrize("name", ["cgs_cell32.npz", "cgs_cell64.npz"])
+def test_cgs_decks_100_steps(orc, device, name):
+    """The reference's cell32 / cell64 decks (fortran90/cell32, fortran90/cell64: one
+    box, nsteps 100) on its CGS initial condition (fortran90/init_data.f90:43-85):
+    after every step dt and the SHA-256 of the interior equal the reference cpu/
+    path's (golden fixture); every ABI array equals the oracle's after the last
+    step (and U, ghosts included, after every step at 32^3)."""
+    g = golden(name)
+    n = int(g["n"])
+    lay = BoxLayout.cube(n)
+    dx = box_dx(n)
+    U = cgs_state(lay)
+    Unew, Q = lay.new_ghosted(NC), lay.new_ghosted(NQ)
+    D, F = lay.new_interior(NC), lay.new_interior(NC)
+    idx = g["sample_idx"]
+    nsteps = int(g["steps"])
+    print(f"CGS deck {n}^3 x {nsteps} steps", flush=True)
+    run = _stepper_run(lay, U.copy(), nsteps, device)
+    for s, (dt, Ud, Unewd, Qd, Dd, Fd) in enumerate(run, 1):
+        assert dt == g["dt"][s - 1], f"dt at step {s}"
+        got = host(Ud)
+        cur = np.ascontiguousarray(lay.interior_of(lay.gview(got, NC)))
+        assert hashlib.sha256(cur.tobytes()).hexdigest() == str(g["sha256"][s - 1]), f"step {s}"
+        assert np.array_equal(cur[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]], g["samples"][s - 1])
+        if n == 32 or s == nsteps:
+            dto = orc.advance(lay, dx, CFL, ETA, ALAM, U, Unew, Q, D, F) if n == 32 else None
+            if n != 32:  # the oracle's 100 steps at once, then every array compared
+                for _ in range(nsteps):
+                    dto = orc.advance(lay, dx, CFL, ETA, ALAM, U, Unew, Q, D, F)
+            assert dt == dto
+            assert np.array_equal(got, U), f"U at step {s}"
+            if s == nsteps:
+                for a, b, nm in ((Unewd, Unew, "Unew"), (Qd, Q, "Q"), (Dd, D, "D"), (Fd, F, "F")):
+                    assert np.array_equal(host(a), b), f"{nm} at step {s}"
+
+
 def test_64_cube_100_steps_every_step(orc, device):
     """64^3, 100 RK3 steps, every conserved field, Unew, Q, D, F and dt compared
     with the oracle after every step."""

@@ -72,3 +72,21 @@ def test_cgs_16_bit_exact():
     out = list(_run(lay, U, 3))
     assert [d for d, _ in out] == list(g["dt"])
     assert np.array_equal(out[-1][1], g["U3"])
+
+
+@pytest.mark.parametrize("name", ["cgs_cell32.npz", "cgs_cell64.npz"])
+def test_cgs_decks_100_steps_bit_exact(name):
+    """The reference's cell32 / cell64 decks (fortran90/cell32, fortran90/cell64: one
+    box, 100 steps) on the CGS initial condition (fortran90/init_data.f90:43-85): the
+    oracle's dt and interior SHA-256 equal the reference's after every step."""
+    from cns_nospec_amd.state import cgs_state
+    g = golden(name)
+    n = int(g["n"])
+    lay = BoxLayout.cube(n)
+    U = cgs_state(lay)
+    assert hashlib.sha256(_interior(lay, U, NC).tobytes()).hexdigest() == str(g["u0_sha256"])
+    idx = g["sample_idx"]
+    for s, (dt, cur) in enumerate(_run(lay, U, int(g["steps"]))):
+        assert dt == g["dt"][s], f"dt at step {s + 1}"
+        assert hashlib.sha256(cur.tobytes()).hexdigest() == str(g["sha256"][s]), f"step {s + 1}"
+        assert np.array_equal(cur[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]], g["samples"][s])
