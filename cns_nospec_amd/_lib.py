@@ -75,6 +75,7 @@ _SIGS = {
     "cns_ctoprim": (_INT, [_I3, _I3, _VP, _VP, _D3, _INT, C.POINTER(C.c_double)]),
     "cns_hypterm": (_INT, [_I3, _I3, _INT, _D3, _VP, _VP, _VP]),
     "cns_diffterm": (_INT, [_I3, _I3, _INT, _D3, _VP, _VP, _DBL, _DBL]),
+    "cns_diffterm_noalloc": (_INT, [_I3, _I3, _INT, _D3, _VP, _VP, _DBL, _DBL]),
     "cns_rk_update": (_INT, [_INT, _I3, _I3, _INT, _VP, _VP, _VP, _VP, _DBL]),
     "cns_ctx_create": (_INT, [C.POINTER(_VP), _I3, _D3, _DBL, _DBL, _DBL]),
     "cns_ctx_destroy": (_INT, [_VP]),

@@ -123,6 +123,38 @@ int thread_scratch(size_t count, double **out)
     return CNS_OK;
 }
 
+// Scratch of cns_diffterm_noalloc: the wz field of the pipelined diffterm (one
+// ghosted component) for boxes whose ghosted component holds at most
+// CNS_NOALLOC_SCRATCH_DOUBLES values.  Each GPU the library runs on holds its own
+// copy until the process ends; every call on that GPU shares it.  In host code the
+// name is only a handle: kernels get the per-device address from
+// hipGetSymbolAddress (noalloc_scratch below).
+__device__ __align__(256) double scratch[CNS_NOALLOC_SCRATCH_DOUBLES];
+
+std::mutex g_scratch_mu;
+double *g_scratch_dev[64] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int noalloc_scratch(double **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(CNS_EINVAL, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(CNS_EHIP, "hipGetSymbolAddress of the no-alloc scratch failed: %s",
+                        hipGetErrorString(e));
+        }
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return CNS_OK;
+}
+
 // The tiled implementation runs the DMA-pipelined kernels (cns_stencil_pipe.hip)
 // when every array is 16-byte aligned, else the register-staged ones
 // (cns_stencil.hip).  CNS_PIPE=0 in the environment selects the latter always.
@@ -349,6 +381,28 @@ int cns_diffterm(const int lo[3], const int hi[3], int ng, const double dx[3], c
     double *wz = nullptr;
     if (g_stencil_impl == CNS_IMPL_TILED && pipe_on({q, difflux})) {
         rc = thread_scratch((size_t)g.cg, &wz);
+        if (rc) return rc;
+    }
+    HIPCHK(stencil_diffterm(g, q, difflux, dxi, eta, alam, wz, t_stream));
+    return CNS_OK;
+}
+
+int cns_diffterm_noalloc(const int lo[3], const int hi[3], int ng, const double dx[3],
+                         const double *q, double *difflux, double eta, double alam)
+{
+    Geom g;
+    int rc = geom_from_bounds(lo, hi, ng, &g);
+    if (rc) return rc;
+    if (!dx || !q || !difflux) return fail(CNS_EINVAL, "NULL argument");
+    if ((size_t)g.cg > (size_t)CNS_NOALLOC_SCRATCH_DOUBLES)
+        return fail(CNS_EINVAL,
+                    "box %dx%dx%d: its ghosted component holds %lld values, above the no-alloc "
+                    "scratch of %d (use cns_diffterm)",
+                    g.n0, g.n1, g.n2, (long long)g.cg, CNS_NOALLOC_SCRATCH_DOUBLES);
+    double dxi[3] = {1.0E0 / dx[0], 1.0E0 / dx[1], 1.0E0 / dx[2]};  // cpu/diffterm.cpp:54-57
+    double *wz = nullptr;
+    if (g_stencil_impl == CNS_IMPL_TILED && pipe_on({q, difflux})) {
+        rc = noalloc_scratch(&wz);
         if (rc) return rc;
     }
     HIPCHK(stencil_diffterm(g, q, difflux, dxi, eta, alam, wz, t_stream));

@@ -84,6 +84,21 @@ int cns_hypterm(const int lo[3], const int hi[3], int ng, const double dx[3],
 int cns_diffterm(const int lo[3], const int hi[3], int ng, const double dx[3],
                  const double *q, double *difflux, double eta, double alam);
 
+/* cns_diffterm for callers that cannot allocate device memory: code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same arguments
+ * and the same bits as cns_diffterm, but its scratch (the wz field of the tiled
+ * kernel, one ghosted component) is a static device array of
+ * CNS_NOALLOC_SCRATCH_DOUBLES doubles (64 MiB) that each GPU holds from the
+ * library's first use until the process ends; nothing is allocated or freed.
+ * Boxes whose ghosted component exceeds it (PAD16(n0+8)*(n1+8)*(n2+8) values;
+ * cubes above 192^3) are rejected with CNS_EINVAL.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (other threads, other streams) race
+ * and must be serialised by the caller.
+ * No counterpart in the reference. */
+#define CNS_NOALLOC_SCRATCH_DOUBLES (8 << 20)
+int cns_diffterm_noalloc(const int lo[3], const int hi[3], int ng, const double dx[3],
+                         const double *q, double *difflux, double eta, double alam);
+
 /* One SSP-RK3 stage update over interior cells (ghosts untouched):
  *   phase 1: unew = u + dt(d+f)
  *   phase 2: unew = 3/4 u + 1/4 (unew + dt(d+f))
