@@ -8,11 +8,11 @@ hipGraph.  Box state is ghost-free ([5][e][e][e] float64, component outermost).
 ``fp32=True`` selects the FP32-flux mode (stencil arithmetic of hypterm and
 diffterm in FP32; state, RK3 accumulation, p/T recovery and dt in FP64).
 
-The tile shape of each box edge comes from ``tuning_gfx950.json``: 32-wide tiles
-and z runs of min(edge, 16) for every edge, so that all boxes share one width class
-(measured on a mixed request stream; edges the table does not list get the same).
-The autotuner below (:func:`autotune`) times every shape per edge on batches of one
-edge, for workloads that are not mixed; its table can replace the file.
+The tile shape of each box edge comes from ``tuning_gfx950.json``, which the
+autotuner below (:func:`autotune`, ``python -m cns_nospec_amd.batch``) writes: it
+times whole tile tables on the served request mix resident in one engine (one width
+for every edge, so each RK stage stays one launch per kernel, and a z run per edge
+bucket).  Edges the table does not list get 32-wide tiles and z runs of min(edge, 16).
 """
 from __future__ import annotations
 
@@ -143,11 +143,17 @@ class BatchEngine:
 
 
 def load_tuning_for(fp32: bool, path: str = TUNING_FILE):
+    """{edge: (width, kz)} of one precision from a tuning file; a key is one edge or an
+    inclusive range "lo-hi"."""
     if not os.path.exists(path):
         return {}
     raw = json.load(open(path))
-    table = raw.get("fp32" if fp32 else "fp64", {})
-    return {int(k): (int(v[0]), int(v[1])) for k, v in table.items()}
+    out = {}
+    for k, v in raw.get("fp32" if fp32 else "fp64", {}).items():
+        lo, _, hi = k.partition("-")
+        for e in range(int(lo), int(hi or lo) + 1):
+            out[e] = (int(v[0]), int(v[1]))
+    return out
 
 
 # ------------------------------------------------------------------ layouts
@@ -169,70 +175,116 @@ def compact_sinusoid(edge: int, seed: int, mach: float = 0.085, kmax: int = 2, d
 
 # ---------------------------------------------------------------- autotuner
 WIDTHS = (16, 32, 64)
+KZS = (8, 16, 32, 64)
+# edge ranges whose z run the tuner chooses separately (the engine admits edges 8..512)
+EDGE_BUCKETS = ((8, 31), (32, 47), (48, 63), (64, 95), (96, 512))
 
 
-def candidate_tiles(edge: int):
-    kzs = sorted({k for k in (8, 16, 32, 64, edge) if k <= edge})
-    return [(w, kz) for w in WIDTHS for kz in kzs if w <= max(16, 2 * edge)]
-
-
-def autotune(edges, fp32: bool, cells_per_trial: int = 24_000_000, steps: int = 3,
-             budget: int = 96 << 30, verbose: bool = False):
-    """Measure every candidate tile shape per box edge and keep the fastest.
-
-    For each edge a batch of identical-edge boxes (about cells_per_trial cells)
-    is stepped with each candidate (width, kz); the rate in cell-steps/s decides.
-    Returns {edge: (width, kz, cell_steps_per_s)}."""
+def _rate(eng, cells: int, steps: int, reps: int) -> float:
+    """Best of `reps` timed windows of `steps` engine steps (after one untimed step that
+    rebuilds and captures the step graph), in cell-steps/s."""
     torch = _torch()
-    out = {}
-    for edge in edges:
-        nbox = max(1, min(512, cells_per_trial // edge ** 3))
-        eng = BatchEngine(min(budget, nbox * box_bytes(edge, fp32) + (64 << 20)), fp32=fp32,
-                          max_boxes=nbox, tuning={})
-        u0 = compact_sinusoid(edge, 7, kmax=2)
-        for _ in range(nbox):
-            eng.add(u0.reshape(-1), edge)
-        best = None
-        for w, kz in candidate_tiles(edge):
-            eng.set_tile(edge, w, kz)
-            eng.step(1)
-            torch.cuda.synchronize()
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            eng.step(steps)
-            b.record()
-            torch.cuda.synchronize()
-            rate = nbox * edge ** 3 * steps / (a.elapsed_time(b) / 1e3)
-            if verbose:
-                print(json.dumps({"autotune": edge, "fp32": fp32, "width": w, "kz": kz,
-                                  "cell_steps_per_s": rate}), flush=True)
-            if best is None or rate > best[2]:
-                best = (w, kz, rate)
-        out[edge] = best
-        eng.close()
-        del u0
-        torch.cuda.empty_cache()
-    return out
+    eng.step(1)
+    torch.cuda.synchronize()
+    best = 0.0
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        eng.step(steps)
+        b.record()
+        torch.cuda.synchronize()
+        best = max(best, cells * steps / (a.elapsed_time(b) / 1e3))
+    return best
 
 
-def save_tuning(results_fp32: dict, results_fp64: dict, path: str = TUNING_FILE):
+def autotune(requests, fp32: bool, widths=WIDTHS, kzs=KZS, buckets=EDGE_BUCKETS, steps: int = 4,
+             reps: int = 2, verbose: bool = False):
+    """Tile table for a mixed workload, timed on the mix itself.
+
+    All `requests` (objects with .edge, and what serve.request_state needs) are resident in
+    one engine, which is what the server runs.  Stage 1: every (width, kz) applied to every
+    edge (one width class, so each RK stage stays one launch per kernel); the fastest is the
+    base.  Stage 2: for each edge bucket in turn, every kz with the base width for the
+    bucket's edges, kept when the whole mix steps faster.  Tables are judged by the mix's
+    aggregate cell-steps/s; tiling never changes results (each cell's arithmetic is the
+    same whichever lane computes it).
+
+    Returns (table {"lo-hi": (width, kz)} per edge bucket, log of the trials)."""
+    from .serve import request_state
+    torch = _torch()
+    eng = BatchEngine(sum(box_bytes(r.edge, fp32) for r in requests) + (256 << 20), fp32=fp32,
+                      max_boxes=len(requests), tuning={})
+    for r in requests:
+        eng.add(request_state(r).reshape(-1), r.edge)
+    present = sorted({r.edge for r in requests})
+    cells = sum(r.edge ** 3 for r in requests)
+    log = []
+
+    def trial(table, label):
+        for e in present:
+            w, kz = table(e)
+            eng.set_tile(e, w, kz)
+        rate = _rate(eng, cells, steps, reps)
+        log.append({"trial": label, "cell_steps_per_s": rate})
+        if verbose:
+            print(json.dumps({"autotune": label, "fp32": fp32, "cell_steps_per_s": rate}), flush=True)
+        return rate
+
+    best, base = 0.0, None
+    for w in widths:
+        for kz in kzs:
+            rate = trial(lambda e, w=w, kz=kz: (w, kz), f"all edges {w} x {kz}")
+            if rate > best:
+                best, base = rate, (w, kz)
+    choice = {bk: base[1] for bk in buckets}
+
+    def bucket_of(e):
+        return next((bk for bk in buckets if bk[0] <= e <= bk[1]), None)
+
+    for bk in buckets:
+        if not any(bk[0] <= e <= bk[1] for e in present):
+            continue
+        for kz in kzs:
+            if kz == choice[bk]:
+                continue
+            trial_choice = dict(choice)
+            trial_choice[bk] = kz
+            rate = trial(lambda e, c=trial_choice: (base[0], c.get(bucket_of(e), base[1])),
+                         f"edges {bk[0]}-{bk[1]} kz {kz}")
+            if rate > best:
+                best, choice = rate, trial_choice
+    eng.close()
+    del eng
+    torch.cuda.empty_cache()
+    table = {f"{bk[0]}-{bk[1]}": (base[0], choice[bk]) for bk in buckets}
+    log.append({"chosen": {k: list(v) for k, v in table.items()}, "cell_steps_per_s": best})
+    return table, log
+
+
+def save_tuning(table_fp32: dict, table_fp64: dict, method: str, logs: dict, path: str = TUNING_FILE):
     doc = {"device": "gfx950 (MI355X)", "generated": time.strftime("%Y-%m-%d"),
-           "fp32": {str(k): [v[0], v[1]] for k, v in sorted(results_fp32.items())},
-           "fp64": {str(k): [v[0], v[1]] for k, v in sorted(results_fp64.items())},
-           "rates": {"fp32": {str(k): v[2] for k, v in results_fp32.items()},
-                     "fp64": {str(k): v[2] for k, v in results_fp64.items()}}}
+           "written_by": "cns_nospec_amd.batch autotune", "method": method,
+           "fp32": {str(k): [v[0], v[1]] for k, v in table_fp32.items()},
+           "fp64": {str(k): [v[0], v[1]] for k, v in table_fp64.items()},
+           "trials": logs}
     with open(path, "w") as f:
         json.dump(doc, f, indent=1)
 
 
 if __name__ == "__main__":
     import argparse
-    ap = argparse.ArgumentParser(description="autotune the batched engine's tile table")
-    ap.add_argument("--edges", default="16,24,32,48,64,96,128")
+    ap = argparse.ArgumentParser(description="autotune the batched engine's tile table on the "
+                                             "served request mix")
+    ap.add_argument("--requests", type=int, default=200, help="boxes of the request mix resident")
+    ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--out", default=TUNING_FILE)
     args = ap.parse_args()
-    edges = [int(x) for x in args.edges.split(",")]
-    r32 = autotune(edges, True, verbose=True)
-    r64 = autotune(edges, False, verbose=True)
-    save_tuning(r32, r64, args.out)
+    from .serve import make_stream
+    reqs = make_stream(args.requests, 0.0, args.seed)
+    t32, l32 = autotune(reqs, True, verbose=True)
+    t64, l64 = autotune(reqs, False, verbose=True)
+    method = (f"{args.requests} requests of the served stream (seed {args.seed}: edges U{{24..128}}) "
+              "resident in one engine; every (width, kz) for all edges, then kz per edge bucket "
+              "with the best width; aggregate cell-steps/s of the mix decides")
+    save_tuning(t32, t64, method, {"fp32": l32, "fp64": l64}, args.out)
     print(json.dumps({"saved": args.out}))

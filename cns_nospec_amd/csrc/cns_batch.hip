@@ -395,6 +395,24 @@ __device__ __forceinline__ void diff_tile(BoxDev *__restrict__ boxes, int b, int
             d_my[zo] = dmy;
             d_mz[zo] = dmz;
             d_E[zo] = alam * (txx + tyy + tzz) + mech;
+#undef CUx
+#undef CUy
+#undef CUz
+#undef CVx
+#undef CVy
+#undef CVz
+#undef CWx
+#undef CWy
+#undef CWz
+#undef CTx
+#undef CTy
+#undef CTz
+#undef VYx
+#undef WZx
+#undef UXy
+#undef WZy
+#undef UXz
+#undef VYz
         }
 #pragma unroll
         for (int m = 0; m < 8; m++) {
@@ -406,13 +424,312 @@ __device__ __forceinline__ void diff_tile(BoxDev *__restrict__ boxes, int b, int
 #undef LVY
 }
 
+// ------------------------------------------------------------- staged diffterm tile
+// The same arithmetic as diff_tile, with the plane data staged by direct-to-LDS loads:
+// each lane moves 4 bytes from its own (periodically wrapped) address straight into a
+// lane-linear LDS slot, no registers involved.  Per plane k the tile needs the centre
+// plane's u, v, w, T and wz (a field written by batch_wz_kernel, so no z neighbours of
+// halo columns are gathered) as full (TY+8) x (TX+8) tiles, and the leading plane
+// k+4's u on the tile rows (for the ux window) and v on the tile columns (vy window).
+// They go into one of two buffers; the loads of plane k+1 are issued right after the
+// barrier that opens plane k, so the compute of plane k hides them.  Two barriers per
+// plane: the opening one (plane k landed, plane k-1 fully consumed) and one between
+// the halo derivatives of phase B and the outputs of phase C.
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <typename FT, int LTX>
+struct DmaGeo {
+    static constexpr int TX = 1 << LTX, TY = kBatchThreads >> LTX, P = TX + 8, R = TY + 8;
+    static constexpr int EPW = 256 / (int)sizeof(FT);  // elements one wave moves (4 B per lane)
+    static constexpr int NW = kBatchThreads / 64;
+    // regions of one buffer: full tiles (centre u, v, w, T, wz), leading u rows, leading v columns
+    static constexpr int CF = R * P, CU = TY * P, CV = R * TX;
+    static constexpr int SF = (CF + EPW - 1) / EPW, SU = (CU + EPW - 1) / EPW, SV = (CV + EPW - 1) / EPW;
+    static constexpr int LF = SF * EPW, LU = SU * EPW, LV = SV * EPW;  // padded lengths
+    static constexpr int NCF = 5;
+    static constexpr int BUF = NCF * LF + LU + LV;
+    // slots per wave (wave w takes slots w, w + NW, ...)
+    static constexpr int WF = (SF + NW - 1) / NW, WU = (SU + NW - 1) / NW, WV = (SV + NW - 1) / NW;
+    static constexpr int UXN = R * TX, VYN = TY * P;  // s_ux: R rows x TX cols; s_vy: TY rows x P cols
+    static constexpr int LDS_ELEMS = 2 * BUF + UXN + VYN;
+};
+
+__device__ __forceinline__ int pmod(int a, int e) { const int r = a % e; return r < 0 ? r + e : r; }
+
+// In-plane offset (y*e + x, periodically wrapped) of element `el` of a region whose
+// row r / column c map to box cell (x0 + c - 4 + cshift, y0 + r - 4 + rshift); elements
+// past the region's end (slot padding) load cell (0, 0) into the padding.
+__device__ __forceinline__ unsigned region_off(int el, int n, int pitch, int rshift, int cshift, int x0,
+                                               int y0, int e)
+{
+    if (el >= n) return 0u;
+    const int r = el / pitch, c = el - r * pitch;
+    return (unsigned)(pmod(y0 + r - 4 + rshift, e) * e + pmod(x0 + c - 4 + cshift, e));
+}
+
+template <typename FT, int LTX>
+__device__ __forceinline__ void diff_tile_dma(BoxDev *__restrict__ boxes, int b, int x0, int y0, int z0,
+                                              int nz, int stage, FT *lds)
+{
+    using KK = K<FT>;
+    using G = DmaGeo<FT, LTX>;
+    constexpr int TXr = G::TX, P = G::P, EPW = G::EPW;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const BoxDev &B = boxes[b];
+    const TileGeo t = tile_geo<LTX>(B.e, x0, y0);
+    const int e = t.e;
+    const unsigned e2 = t.e2, e3 = t.e3, own = t.own;
+    const int row = tid >> LTX, col = tid & (TXr - 1);
+    const int me = (row + 4) * P + col + 4;  // own cell in a full tile
+    const GArr<FT> qi = garr<FT>(B.Q[stage - 1]);
+    const GArr<FT> q_u = qi, q_v = qi + e3, q_w = qi + 2 * (size_t)e3, q_t = qi + 4 * (size_t)e3;
+    const GArr<FT> wzf = garr<FT>(B.WZ);
+    const GArr<FT> d_mx = garr<FT>(B.D), d_my = d_mx + e3, d_mz = d_mx + 2 * (size_t)e3,
+                   d_E = d_mx + 3 * (size_t)e3;
+    const FT dxi0 = (FT)B.dxi[0], dxi1 = (FT)B.dxi[1], dxi2 = (FT)B.dxi[2];
+    const FT dxi0s = (FT)(B.dxi[0] * B.dxi[0]), dxi1s = (FT)(B.dxi[1] * B.dxi[1]),
+             dxi2s = (FT)(B.dxi[2] * B.dxi[2]);
+    const FT eta = (FT)B.eta, alam = (FT)B.alam;
+    FT *const s_ux = lds + 2 * G::BUF, *const s_vy = s_ux + G::UXN;
+
+    // Per-lane in-plane byte offsets of this wave's slots (the same for every plane).
+    // A 4-byte lane of an 8-byte element moves its low (even lane) or high half.
+    constexpr int SUB = (int)sizeof(FT) / 4;
+    const int lel = lane / SUB;
+    const unsigned half = (unsigned)(lane % SUB) * 4u;
+    unsigned of[G::WF], ou[G::WU], ov[G::WV];
+#pragma unroll
+    for (int j = 0; j < G::WF; j++)
+        of[j] = region_off((wave + j * G::NW) * EPW + lel, G::CF, P, 0, 0, x0, y0, e) * sizeof(FT) + half;
+#pragma unroll
+    for (int j = 0; j < G::WU; j++)
+        ou[j] = region_off((wave + j * G::NW) * EPW + lel, G::CU, P, 4, 0, x0, y0, e) * sizeof(FT) + half;
+#pragma unroll
+    for (int j = 0; j < G::WV; j++)
+        ov[j] = region_off((wave + j * G::NW) * EPW + lel, G::CV, TXr, 0, 4, x0, y0, e) * sizeof(FT) + half;
+
+    const char GAS *const src[G::NCF] = {(const char GAS *)q_u.p, (const char GAS *)q_v.p,
+                                         (const char GAS *)q_w.p, (const char GAS *)q_t.p,
+                                         (const char GAS *)wzf.p};
+    // plane k (centre fields) and its leading plane k+4 (u rows, v columns) into buffer `buf`
+    auto issue = [&](int k, FT *buf) {
+        const unsigned pc = (unsigned)k * e2 * (unsigned)sizeof(FT);
+        const unsigned pl = (unsigned)pmod(k + 4, e) * e2 * (unsigned)sizeof(FT);
+#pragma unroll
+        for (int j = 0; j < G::WF; j++) {
+            const int s = wave + j * G::NW;
+            if (s < G::SF) {
+#pragma unroll
+                for (int f = 0; f < G::NCF; f++)
+                    __builtin_amdgcn_global_load_lds((const void *)(src[f] + (pc + of[j])),
+                                                     (lds_void *)(buf + f * G::LF + s * EPW), 4, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < G::WU; j++) {
+            const int s = wave + j * G::NW;
+            if (s < G::SU)
+                __builtin_amdgcn_global_load_lds((const void *)(src[0] + (pl + ou[j])),
+                                                 (lds_void *)(buf + G::NCF * G::LF + s * EPW), 4, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < G::WV; j++) {
+            const int s = wave + j * G::NW;
+            if (s < G::SV)
+                __builtin_amdgcn_global_load_lds((const void *)(src[1] + (pl + ov[j])),
+                                                 (lds_void *)(buf + G::NCF * G::LF + G::LU + s * EPW), 4, 0, 0);
+        }
+    };
+    issue(z0, lds);
+
+    // windows: plane k+m at [4+m]; the first 8 planes and their ux, vy straight from
+    // global memory (the same expressions as the per-plane path)
+    FT wu[9], wv[9], ww[9], wt[9], wux[9], wvy[9];
+#pragma unroll 1
+    for (int m = 0; m < 8; m++) {
+        const unsigned pl = (unsigned)pmod(z0 - 4 + m, e) * e2;
+        const unsigned rowo = pl + (unsigned)t.yo * e;
+#define GUX(mm) q_u[rowo + (unsigned)wrapc(t.xo + (mm), e)]
+#define GVY(mm) q_v[pl + (unsigned)wrapc(t.yo + (mm), e) * e + (unsigned)t.xo]
+        const FT uxm = KD1(GUX) * dxi0, vym = KD1(GVY) * dxi1;
+#undef GUX
+#undef GVY
+#pragma unroll
+        for (int mm = 0; mm < 7; mm++) {
+            wux[mm] = wux[mm + 1];
+            wvy[mm] = wvy[mm + 1];
+        }
+        wux[7] = uxm;
+        wvy[7] = vym;
+    }
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const unsigned o = (unsigned)pmod(z0 - 4 + m, e) * e2 + own;
+        wu[m] = q_u[o]; wv[m] = q_v[o]; ww[m] = q_w[o]; wt[m] = q_t[o];
+    }
+    FT nw, nt;  // own w, T of the next leading plane
+    {
+        const unsigned o = (unsigned)pmod(z0 + 4, e) * e2 + own;
+        nw = q_w[o];
+        nt = q_t[o];
+    }
+
+    for (int k = z0; k < z0 + nz; k++) {
+        FT *const bc = lds + ((k - z0) & 1) * G::BUF;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (k + 1 < z0 + nz) issue(k + 1, lds + (((k - z0) & 1) ^ 1) * G::BUF);
+        const FT *s_cu = bc, *s_cv = bc + G::LF, *s_cw = bc + 2 * G::LF, *s_ct = bc + 3 * G::LF,
+                 *s_wz = bc + 4 * G::LF, *s_lu = bc + 5 * G::LF, *s_lv = bc + 5 * G::LF + G::LU;
+        const int mlu = row * P + col + 4, mlv = (row + 4) * TXr + col;
+        wu[8] = s_lu[mlu];
+        wv[8] = s_lv[mlv];
+        ww[8] = nw;
+        wt[8] = nt;
+        if (k + 1 < z0 + nz) {
+            const unsigned o = (unsigned)pmod(k + 5, e) * e2 + own;
+            nw = q_w[o];
+            nt = q_t[o];
+        }
+        // phase B: leading-plane ux, vy into the windows; centre ux on the y-halo rows,
+        // vy on the x-halo columns (cpu/diffterm.cpp:77-81, :110-114)
+#define LUX(mm) s_lu[mlu + (mm)]
+#define LVY(mm) s_lv[mlv + (mm) * TXr]
+        wux[8] = KD1(LUX) * dxi0;
+        wvy[8] = KD1(LVY) * dxi1;
+#undef LUX
+#undef LVY
+        s_ux[(row + 4) * TXr + col] = wux[4];
+        for (int h = tid; h < 8 * TXr; h += kBatchThreads) {
+            const int rr = h >> LTX, r = rr < 4 ? rr : rr + G::TY, c = h & (TXr - 1);
+#define SUX(mm) s_cu[r * P + c + 4 + (mm)]
+            s_ux[r * TXr + c] = KD1(SUX) * dxi0;
+#undef SUX
+        }
+        s_vy[row * P + col + 4] = wvy[4];
+        for (int h = tid; h < 8 * G::TY; h += kBatchThreads) {
+            const int rr = h >> 3, cc = h & 7, c = cc < 4 ? cc : cc + TXr;
+#define SVY(mm) s_cv[(rr + 4 + (mm)) * P + c]
+            s_vy[rr * P + c] = KD1(SVY) * dxi1;
+#undef SVY
+        }
+        __syncthreads();
+        // phase C (cpu/diffterm.cpp:155-311)
+        if (t.active) {
+#define CUx(m) s_cu[me + (m)]
+#define CUy(m) s_cu[me + (m) * P]
+#define CUz(m) wu[4 + (m)]
+#define CVx(m) s_cv[me + (m)]
+#define CVy(m) s_cv[me + (m) * P]
+#define CVz(m) wv[4 + (m)]
+#define CWx(m) s_cw[me + (m)]
+#define CWy(m) s_cw[me + (m) * P]
+#define CWz(m) ww[4 + (m)]
+#define CTx(m) s_ct[me + (m)]
+#define CTy(m) s_ct[me + (m) * P]
+#define CTz(m) wt[4 + (m)]
+#define VYx(m) s_vy[row * P + col + 4 + (m)]
+#define WZx(m) s_wz[me + (m)]
+#define UXy(m) s_ux[(row + 4 + (m)) * TXr + col]
+#define WZy(m) s_wz[me + (m) * P]
+#define UXz(m) wux[4 + (m)]
+#define VYz(m) wvy[4 + (m)]
+            const unsigned zo = (unsigned)k * e2 + own;
+            const FT uxx = KD2(CUx) * dxi0s, uyy = KD2(CUy) * dxi1s, uzz = KD2(CUz) * dxi2s;
+            const FT vyx = KD1(VYx) * dxi0, wzx = KD1(WZx) * dxi0;
+            const FT dmx = eta * (KK::FourThirds * uxx + uyy + uzz + KK::OneThird * (vyx + wzx));
+            FENCE();
+            const FT vxx = KD2(CVx) * dxi0s, vyy = KD2(CVy) * dxi1s, vzz = KD2(CVz) * dxi2s;
+            const FT uxy = KD1(UXy) * dxi1, wzy = KD1(WZy) * dxi1;
+            const FT dmy = eta * (vxx + KK::FourThirds * vyy + vzz + KK::OneThird * (uxy + wzy));
+            FENCE();
+            const FT wxx = KD2(CWx) * dxi0s, wyy = KD2(CWy) * dxi1s, wzz = KD2(CWz) * dxi2s;
+            const FT uxz = KD1(UXz) * dxi2, vyz = KD1(VYz) * dxi2;
+            const FT dmz = eta * (wxx + wyy + KK::FourThirds * wzz + KK::OneThird * (uxz + vyz));
+            FENCE();
+            const FT txx = KD2(CTx) * dxi0s, tyy = KD2(CTy) * dxi1s, tzz = KD2(CTz) * dxi2s;
+            FENCE();
+            const FT ux = wux[4], vy = wvy[4], wz = s_wz[me];
+            const FT uy = KD1(CUy) * dxi1, uz = KD1(CUz) * dxi2;
+            const FT vx = KD1(CVx) * dxi0, vz = KD1(CVz) * dxi2;
+            const FT wx = KD1(CWx) * dxi0, wy = KD1(CWy) * dxi1;
+            const FT divu = KK::TwoThirds * (ux + vy + wz);
+            const FT tauxx = KK::Two * ux - divu, tauyy = KK::Two * vy - divu;
+            const FT tauzz = KK::Two * wz - divu;
+            const FT tauxy = uy + vx, tauxz = uz + wx, tauyz = vz + wy;
+            FT mech = tauxx * ux + tauyy * vy + tauzz * wz + (tauxy * tauxy) + (tauxz * tauxz)
+                      + (tauyz * tauyz);
+            mech = eta * mech + dmx * wu[4] + dmy * wv[4] + dmz * ww[4];
+            d_mx[zo] = dmx;
+            d_my[zo] = dmy;
+            d_mz[zo] = dmz;
+            d_E[zo] = alam * (txx + tyy + tzz) + mech;
+#undef CUx
+#undef CUy
+#undef CUz
+#undef CVx
+#undef CVy
+#undef CVz
+#undef CWx
+#undef CWy
+#undef CWz
+#undef CTx
+#undef CTy
+#undef CTz
+#undef VYx
+#undef WZx
+#undef UXy
+#undef WZy
+#undef UXz
+#undef VYz
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            wu[m] = wu[m + 1]; wv[m] = wv[m + 1]; ww[m] = ww[m + 1]; wt[m] = wt[m + 1];
+            wux[m] = wux[m + 1]; wvy[m] = wvy[m + 1];
+        }
+    }
+}
+
+// wz = D1_z(w) * dxinv(3) of the stage's primitives (cpu/diffterm.cpp:127-151, the same
+// expression as diff_tile's) for the cells of one tile's columns and z run.
+template <typename FT, int LTX>
+__device__ __forceinline__ void wz_tile(BoxDev *__restrict__ boxes, int b, int x0, int y0, int z0, int nz,
+                                        int stage)
+{
+    using KK = K<FT>;
+    const BoxDev &B = boxes[b];
+    const TileGeo t = tile_geo<LTX>(B.e, x0, y0);
+    if (!t.active) return;
+    const int e = t.e;
+    const unsigned e2 = t.e2, own = t.own;
+    const GArr<FT> q_w = garr<FT>(B.Q[stage - 1]) + 2 * (size_t)t.e3;
+    const GArr<FT> wzf = garr<FT>(B.WZ);
+    const FT dxi2 = (FT)B.dxi[2];
+    FT w[9];
+#pragma unroll
+    for (int m = 0; m < 8; m++) w[m] = q_w[(unsigned)pmod(z0 - 4 + m, e) * e2 + own];
+    for (int k = z0; k < z0 + nz; k++) {
+        w[8] = q_w[(unsigned)pmod(k + 4, e) * e2 + own];
+#define WZW(mm) w[4 + (mm)]
+        wzf[(unsigned)k * e2 + own] = KD1(WZW) * dxi2;
+#undef WZW
+#pragma unroll
+        for (int m = 0; m < 8; m++) w[m] = w[m + 1];
+    }
+}
+
 // ------------------------------------------------------------- hypterm+update tile
 // cpu/hypterm.cpp:11-209 on the stage state, then per cell the SSP-RK3 update in
 // FP64 with the tile's D (cpu/advance.cpp:82, :114-116, :148-150), ctoprim of the
 // result into the next stage's primitives, and in stage 3 the Courant maximum.
 // Returns the lane's Courant maximum (bits; stage 3 only).  The stage is a template
 // parameter: stage 1 then has no base operand at all (its base is the stage state).
-template <typename FT, int LTX, int stage>
+// KEEP (FP32 flux only): the FP64 conserved values loaded as the z window's leading
+// plane stay in a 5-plane register FIFO until that plane is the centre, where the
+// RK update needs them, instead of being loaded again four planes later.
+template <typename FT, int LTX, int stage, bool KEEP>
 __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxes, int b, int x0,
                                                        int y0, int z0, int nz, FT *lds)
 {
@@ -455,25 +772,35 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
     // whose z taps would otherwise be 32 global loads per cell per plane that mostly
     // miss L2 (the FP64 mode has no registers to spare for them).
     constexpr bool CWIN = sizeof(FT) == 4;
+    constexpr bool FIFO = KEEP && CWIN;
     FT ww[9], wp[9], wmx[9], wmy[9], wmz[9], wE[9];
+    double fmx[5], fmy[5], fmz[5], fE[5];  // FIFO: planes k .. k+4 (FIFO only)
 #pragma unroll
     for (int m = 0; m < 8; m++) {
         const int o = wrapc(z0 - 4 + m, e) * e2 + own;
         ww[m] = q_w[o]; wp[m] = q_p[o];
-        if (CWIN) {
+        if (FIFO && m >= 4) {
+            fmx[m - 4] = c_mx[o]; fmy[m - 4] = c_my[o]; fmz[m - 4] = c_mz[o]; fE[m - 4] = c_E[o];
+            wmx[m] = (FT)fmx[m - 4]; wmy[m] = (FT)fmy[m - 4]; wmz[m] = (FT)fmz[m - 4]; wE[m] = (FT)fE[m - 4];
+        } else if (CWIN) {
             wmx[m] = (FT)c_mx[o]; wmy[m] = (FT)c_my[o]; wmz[m] = (FT)c_mz[o]; wE[m] = (FT)c_E[o];
         }
     }
-    // prefetched: next leading plane (w, p) and next centre (u, v, conserved)
+    // prefetched: next leading plane (w, p; conserved into the FIFO or as the flux type)
+    // and next centre (u, v, and the conserved values when there is no FIFO)
     FT nw, np, nu, nv, lmx = 0, lmy = 0, lmz = 0, lE = 0;
-    double nmx, nmy, nmz, nE;
+    double nmx = 0.0, nmy = 0.0, nmz = 0.0, nE = 0.0;
     {
         const int o = wrapc(z0 + 4, e) * e2 + own, oc = z0 * e2 + own;
         nw = q_w[o]; np = q_p[o];
         nu = q_u[oc]; nv = q_v[oc];
-        nmx = c_mx[oc]; nmy = c_my[oc]; nmz = c_mz[oc]; nE = c_E[oc];
-        if (CWIN) {
-            lmx = (FT)c_mx[o]; lmy = (FT)c_my[o]; lmz = (FT)c_mz[o]; lE = (FT)c_E[o];
+        if (FIFO) {
+            fmx[4] = c_mx[o]; fmy[4] = c_my[o]; fmz[4] = c_mz[o]; fE[4] = c_E[o];
+        } else {
+            nmx = c_mx[oc]; nmy = c_my[oc]; nmz = c_mz[oc]; nE = c_E[oc];
+            if (CWIN) {
+                lmx = (FT)c_mx[o]; lmy = (FT)c_my[o]; lmz = (FT)c_mz[o]; lE = (FT)c_E[o];
+            }
         }
     }
     unsigned long long cmax = 0ull;
@@ -481,12 +808,22 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
         const int cp = k * e2;
         const int zo = cp + own;
         ww[8] = nw; wp[8] = np;
-        if (CWIN) {
+        if (FIFO) {
+            wmx[8] = (FT)fmx[4]; wmy[8] = (FT)fmy[4]; wmz[8] = (FT)fmz[4]; wE[8] = (FT)fE[4];
+        } else if (CWIN) {
             wmx[8] = lmx; wmy[8] = lmy; wmz[8] = lmz; wE[8] = lE;
         }
+        const double cmx = FIFO ? fmx[0] : nmx, cmy = FIFO ? fmy[0] : nmy;
+        const double cmz = FIFO ? fmz[0] : nmz, cE = FIFO ? fE[0] : nE;
+        if (FIFO) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                fmx[m] = fmx[m + 1]; fmy[m] = fmy[m + 1]; fmz[m] = fmz[m + 1]; fE[m] = fE[m + 1];
+            }
+        }
         const FT u_own = nu, v_own = nv;
-        const FT mx_own = (FT)nmx, my_own = (FT)nmy, mz_own = (FT)nmz, E_own = (FT)nE;
-        const double cur_c[5] = {0.0, nmx, nmy, nmz, nE};
+        const FT mx_own = (FT)cmx, my_own = (FT)cmy, mz_own = (FT)cmz, E_own = (FT)cE;
+        const double cur_c[5] = {0.0, cmx, cmy, cmz, cE};
         __syncthreads();
         s_mx[me] = mx_own; s_my[me] = my_own; s_mz[me] = mz_own; s_E[me] = E_own; s_p[me] = wp[4];
         s_u[me] = u_own; s_v[me] = v_own;
@@ -509,9 +846,13 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
             const int o = wrapc(k + 5, e) * e2 + own, oc = zo + e2;
             nw = q_w[o]; np = q_p[o];
             nu = q_u[oc]; nv = q_v[oc];
-            nmx = c_mx[oc]; nmy = c_my[oc]; nmz = c_mz[oc]; nE = c_E[oc];
-            if (CWIN) {
-                lmx = (FT)c_mx[o]; lmy = (FT)c_my[o]; lmz = (FT)c_mz[o]; lE = (FT)c_E[o];
+            if (FIFO) {
+                fmx[4] = c_mx[o]; fmy[4] = c_my[o]; fmz[4] = c_mz[o]; fE[4] = c_E[o];
+            } else {
+                nmx = c_mx[oc]; nmy = c_my[oc]; nmz = c_mz[oc]; nE = c_E[oc];
+                if (CWIN) {
+                    lmx = (FT)c_mx[o]; lmy = (FT)c_my[o]; lmz = (FT)c_mz[o]; lE = (FT)c_E[o];
+                }
             }
         }
         double rkd[4], rkb[5], rho_c = 0.0;
@@ -631,6 +972,32 @@ __device__ __forceinline__ int xcd_rank(int b, int nb)
     return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
+// The wz field of a stage, over the same tile list as its diffterm launch.
+template <typename FT, int LTX>
+__global__ void __launch_bounds__(kBatchThreads)
+batch_wz_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles, int stage)
+{
+    const int ti = xcd_rank(blockIdx.x, gridDim.x);
+    if (ti >= ntiles) return;
+    const TileArgs a = read_tile(tiles, ti);
+    if (box_stopped(boxes[a.b])) return;  // uniform over the workgroup
+    wz_tile<FT, LTX>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, stage);
+}
+
+// Staged diffterm: LDS-limited to 4 workgroups per CU (FP32, 32-wide tiles: 35 KB and
+// at most 128 VGPRs), 2 in FP64.
+template <typename FT, int LTX>
+__global__ void __launch_bounds__(kBatchThreads, sizeof(FT) == 4 ? 4 : 2)
+batch_diff_dma_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles, int stage)
+{
+    __shared__ FT lds[DmaGeo<FT, LTX>::LDS_ELEMS];
+    const int ti = xcd_rank(blockIdx.x, gridDim.x);
+    if (ti >= ntiles) return;
+    const TileArgs a = read_tile(tiles, ti);
+    if (box_stopped(boxes[a.b])) return;  // uniform over the workgroup
+    diff_tile_dma<FT, LTX>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, stage, lds);
+}
+
 template <typename FT, int LTX>
 __global__ void __launch_bounds__(kBatchThreads, 2)
 batch_diff_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles, int stage)
@@ -645,7 +1012,7 @@ batch_diff_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles,
 
 // The FP32 stage-1 tile (no base operand, no Courant reduction) fits 3 waves per SIMD
 // without spilling; the others 2.
-template <typename FT, int LTX, int stage>
+template <typename FT, int LTX, int stage, bool KEEP>
 __global__ void __launch_bounds__(kBatchThreads, (sizeof(FT) == 4 && stage == 1) ? 3 : 2)
 batch_hyp_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles)
 {
@@ -654,24 +1021,35 @@ batch_hyp_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, 
     if (ti >= ntiles) return;
     const TileArgs a = read_tile(tiles, ti);
     if (box_stopped(boxes[a.b])) return;  // uniform over the workgroup
-    const unsigned long long cm = hyp_tile<FT, LTX, stage>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, lds);
+    const unsigned long long cm = hyp_tile<FT, LTX, stage, KEEP>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, lds);
     if (stage == 3) {
         const unsigned long long m = block_max_u64(cm);
         if (threadIdx.x == 0) atomicMax(&boxes[a.b].acc, m);
     }
 }
 
+enum Pass { PASS_WZ, PASS_DIFF, PASS_DIFF_STAGED, PASS_HYP, PASS_HYP_KEEP };
+
 template <typename FT, int LTX>
-hipError_t stage_class(BoxDev *boxes, const TileDev *tiles, int n, int stage, bool diff, hipStream_t s)
+hipError_t stage_class(BoxDev *boxes, const TileDev *tiles, int n, int stage, Pass pass, hipStream_t s)
 {
-    if (diff)
-        batch_diff_kernel<FT, LTX><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n, stage);
-    else
+    switch (pass) {
+    case PASS_WZ: batch_wz_kernel<FT, LTX><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n, stage); break;
+    case PASS_DIFF: batch_diff_kernel<FT, LTX><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n, stage); break;
+    case PASS_DIFF_STAGED:
+        batch_diff_dma_kernel<FT, LTX><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n, stage);
+        break;
+    case PASS_HYP_KEEP:  // stages 2 and 3 of the FP32 flux mode
+        if (stage == 2) batch_hyp_kernel<float, LTX, 2, true><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
+        else batch_hyp_kernel<float, LTX, 3, true><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
+        break;
+    default:
         switch (stage) {
-        case 1: batch_hyp_kernel<FT, LTX, 1><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n); break;
-        case 2: batch_hyp_kernel<FT, LTX, 2><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n); break;
-        default: batch_hyp_kernel<FT, LTX, 3><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n); break;
+        case 1: batch_hyp_kernel<FT, LTX, 1, false><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n); break;
+        case 2: batch_hyp_kernel<FT, LTX, 2, false><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n); break;
+        default: batch_hyp_kernel<FT, LTX, 3, false><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n); break;
         }
+    }
     return hipGetLastError();
 }
 
@@ -697,23 +1075,29 @@ hipError_t launch_batch_finalize(BoxDev *boxes, const int *ids, int n, int count
 }
 
 hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int start[3],
-                              const int count[3], int stage, bool fp32, hipStream_t s)
+                              const int count[3], int stage, bool fp32, bool staged_diff,
+                              bool keep_centre, hipStream_t s)
 {
-    for (int pass = 0; pass < 2; pass++) {  // all diffterm launches, then all hypterm launches
-        const bool diff = pass == 0;
+    // every class's wz field (staged diffterm only), then every diffterm, then every
+    // hypterm + update: each pass reads what the one before wrote on other tiles' cells
+    const bool keep = keep_centre && fp32 && stage != 1;
+    const Pass passes[3] = {PASS_WZ, staged_diff ? PASS_DIFF_STAGED : PASS_DIFF,
+                            keep ? PASS_HYP_KEEP : PASS_HYP};
+    for (const Pass pass : passes) {
+        if (pass == PASS_WZ && !staged_diff) continue;
         for (int c = 0; c < 3; c++) {
             const int n = count[c];
             if (n <= 0) continue;
             const TileDev *t = tiles + start[c];
             hipError_t e;
             if (fp32)
-                e = c == 0 ? stage_class<float, 4>(boxes, t, n, stage, diff, s)
-                  : c == 1 ? stage_class<float, 5>(boxes, t, n, stage, diff, s)
-                           : stage_class<float, 6>(boxes, t, n, stage, diff, s);
+                e = c == 0 ? stage_class<float, 4>(boxes, t, n, stage, pass, s)
+                  : c == 1 ? stage_class<float, 5>(boxes, t, n, stage, pass, s)
+                           : stage_class<float, 6>(boxes, t, n, stage, pass, s);
             else
-                e = c == 0 ? stage_class<double, 4>(boxes, t, n, stage, diff, s)
-                  : c == 1 ? stage_class<double, 5>(boxes, t, n, stage, diff, s)
-                           : stage_class<double, 6>(boxes, t, n, stage, diff, s);
+                e = c == 0 ? stage_class<double, 4>(boxes, t, n, stage, pass, s)
+                  : c == 1 ? stage_class<double, 5>(boxes, t, n, stage, pass, s)
+                           : stage_class<double, 6>(boxes, t, n, stage, pass, s);
             if (e != hipSuccess) return e;
         }
     }

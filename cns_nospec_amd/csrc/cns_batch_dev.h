@@ -21,6 +21,7 @@ struct BoxDev {
     double *S2;     // U^{2/3}
     void *Q[3];     // primitives (u, v, w, p - pref, T) of U^n, U^{1/3}, U^{2/3}
     void *D;        // diffterm of the current stage (mx, my, mz, E), flux type
+    void *WZ;       // wz = d(w)/dz of the current stage's primitives, flux type
     unsigned long long acc;  // Courant maximum (bits) of the state the next step starts from
     double dt;      // dt of the current step
     double courno;  // its Courant number
@@ -48,8 +49,12 @@ hipError_t launch_batch_finalize(BoxDev *boxes, const int *box_ids, int nbox, in
                                  hipStream_t s);
 // One RK stage over the tile list: the list holds class 0, 1, 2 tiles in the
 // ranges [start[c], start[c]+count[c]); each class gets a diffterm launch, then a
-// hypterm + update launch, of one workgroup per tile.
+// hypterm + update launch, of one workgroup per tile.  staged_diff selects the
+// diffterm whose planes are staged by direct-to-LDS loads, preceded by a wz-field
+// launch per class; keep_centre the FP32 hypterm + update of stages 2 and 3 that keeps
+// its window's FP64 leading values for the centre plane.
 hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int start[3],
-                              const int count[3], int stage, bool fp32, hipStream_t s);
+                              const int count[3], int stage, bool fp32, bool staged_diff,
+                              bool keep_centre, hipStream_t s);
 
 }  // namespace cns

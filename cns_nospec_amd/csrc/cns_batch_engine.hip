@@ -10,6 +10,7 @@
 // resident set or the tile table changes.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <new>
@@ -110,6 +111,16 @@ struct cns_batch {
     std::map<int, std::pair<int, int>> tune;  // edge -> (width, kz)
     int cls_start[kTileClasses] = {0, 0, 0}, cls_count[kTileClasses] = {0, 0, 0};
     int nresident = 0, ntiles = 0;
+    // diffterm with its planes staged by direct-to-LDS loads (and a wz-field pass), the
+    // default; CNS_BATCH_DIFF=0 in the environment at creation selects the tile that
+    // gathers its halos through registers.  Both give the same bits.
+    bool staged_diff = true;
+    // FP32 flux, stages 2 and 3: CNS_BATCH_KEEP=1 selects the hypterm tile that keeps
+    // its z window's FP64 leading values until they are the centre instead of loading
+    // them again (same bits; its 190-200 VGPRs allow 2 workgroups per CU instead of 3,
+    // which cost more than the 32 bytes per cell it saves: 4.66 vs 4.73 G cell-steps/s
+    // on the request mix).
+    bool keep_centre = false;
     bool dirty = true;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -128,8 +139,9 @@ size_t box_bytes(int e, bool fp32)
 {
     const size_t e3 = (size_t)e * e * e;
     const size_t ft = fp32 ? 4 : 8;
-    // U, S1, S2 (5 doubles each), Q0..Q2 (5 flux-type values each), D (4)
-    return 3 * align_up(5 * e3 * 8) + 3 * align_up(5 * e3 * ft) + align_up(4 * e3 * ft);
+    // U, S1, S2 (5 doubles each), Q0..Q2 (5 flux-type values each), D (4), wz (1)
+    return 3 * align_up(5 * e3 * 8) + 3 * align_up(5 * e3 * ft) + align_up(4 * e3 * ft) +
+           align_up(e3 * ft);
 }
 
 void destroy_graph(cns_batch *b)
@@ -200,6 +212,7 @@ int rebuild(cns_batch *b, hipStream_t s)
         hipError_t e = hipSuccess;
         for (int stage = 1; stage <= 3 && e == hipSuccess; stage++)
             e = launch_batch_stage(b->d_boxes, b->d_tiles, b->cls_start, b->cls_count, stage, b->fp32,
+                                   b->staged_diff, b->keep_centre,
                                    b->cap_stream);
         if (e == hipSuccess) e = launch_batch_finalize(b->d_boxes, b->d_ids, b->nresident, 1, b->cap_stream);
         hipGraph_t g = nullptr;
@@ -239,6 +252,8 @@ int cns_batch_create(cns_batch **out, size_t hbm_budget, int flux_fp32, int max_
     if (!b) return fail(CNS_ENOMEM, "host allocation failed");
     b->fp32 = flux_fp32 != 0;
     b->max_boxes = max_boxes;
+    if (const char *env = getenv("CNS_BATCH_DIFF")) b->staged_diff = atoi(env) != 0;
+    if (const char *env = getenv("CNS_BATCH_KEEP")) b->keep_centre = atoi(env) != 0;
     int rc = CNS_OK;
     if (hipGetDevice(&b->device) != hipSuccess) rc = fail(CNS_EHIP, "hipGetDevice failed");
     if (!rc && hbm_budget && hipMalloc((void **)&b->pool_mem, hbm_budget) != hipSuccess) {
@@ -332,6 +347,8 @@ int cns_batch_add(cns_batch *b, int edge, double dx, double cfl, double eta, dou
         p += align_up(5 * e3 * ft);
     }
     r.D = p;
+    p += align_up(4 * e3 * ft);
+    r.WZ = p;
     int rc = CNS_OK;
     if (hipMemcpyAsync(b->d_boxes + id, &r, sizeof r, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(r.U, u0, 5 * e3 * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess ||

@@ -191,3 +191,66 @@ def test_nonphysical_box_fails_alone(device):
     assert np.array_equal(mix.read(g).cpu().numpy(), ref.read(r).cpu().numpy())
     ref.close()
     mix.close()
+
+
+@pytest.mark.parametrize("fp32", [True, False])
+def test_engine_variants_agree(device, monkeypatch, fp32):
+    """The engine's kernel variants compute the same expressions on the same values, so
+    they give the same bits, for every tile width and partial tiles: the diffterm tile
+    whose planes are staged by direct-to-LDS loads (with a wz field) against the one that
+    gathers its halos through registers, and the FP32 hypterm tile that keeps its z
+    window's FP64 leading values for the centre plane against the one that loads them
+    again."""
+    edges = (16, 20, 24, 40)
+    init = {e: compact_sinusoid(e, 90 + e, mach=0.25, kmax=3) for e in edges}
+    tiles = {16: (16, 5), 20: (32, 7), 24: (64, 24), 40: (32, 16)}
+    out = []
+    for diff, keep in (("1", "1"), ("1", "0"), ("0", "0")):
+        monkeypatch.setenv("CNS_BATCH_DIFF", diff)
+        monkeypatch.setenv("CNS_BATCH_KEEP", keep)
+        eng = BatchEngine(2 << 30, fp32=fp32, max_boxes=8, tuning=tiles)
+        ids = {e: eng.add(init[e].reshape(-1), e) for e in edges}
+        eng.step(6)
+        out.append({e: eng.read(ids[e]).cpu().numpy() for e in edges})
+        eng.close()
+    for variant in out[1:]:
+        for e in edges:
+            assert np.array_equal(variant[e], out[0][e]), f"edge {e}"
+
+
+def test_autotune_table_loads_and_keeps_bits(device, tmp_path):
+    """The autotuner on a two-edge mix: its table uses one width for every bucket and
+    kz values from the candidates, survives the file round trip (every edge of every
+    bucket), is what the engine then runs, and gives the same bits as the default
+    tiles in both precisions."""
+    from cns_nospec_amd.batch import autotune, load_tuning_for, save_tuning
+    from cns_nospec_amd.serve import Request, request_state
+    reqs = [Request(i, 0.0, e, 5, 300 + i, 0.2) for i, e in enumerate((16, 24, 16, 24))]
+    buckets = ((8, 20), (21, 40))
+    tables, logs = {}, {}
+    for fp32 in (True, False):
+        tables[fp32], logs[fp32] = autotune(reqs, fp32, widths=(16, 32), kzs=(8, 16),
+                                            buckets=buckets, steps=2, reps=1)
+        t = tables[fp32]
+        assert set(t) == {"8-20", "21-40"}
+        assert len({w for w, _ in t.values()}) == 1 and all(w in (16, 32) for w, _ in t.values())
+        assert all(kz in (8, 16) for _, kz in t.values())
+        assert len(logs[fp32]) == 2 * 2 + 2 * 1 + 1  # uniform trials, bucket trials, choice
+    path = str(tmp_path / "tuning.json")
+    save_tuning(tables[True], tables[False], "two-edge test mix", {"fp32": logs[True], "fp64": logs[False]},
+                path)
+    for fp32 in (True, False):
+        loaded = load_tuning_for(fp32, path)
+        assert set(loaded) == set(range(8, 41))
+        assert loaded[16] == tuple(tables[fp32]["8-20"]) and loaded[24] == tuple(tables[fp32]["21-40"])
+        out = []
+        for tuning in (loaded, {}):
+            eng = BatchEngine(1 << 30, fp32=fp32, max_boxes=8, tuning=tuning)
+            if tuning:
+                assert eng.get_tile(24) == loaded[24]
+            ids = [eng.add(request_state(r).reshape(-1), r.edge) for r in reqs]
+            eng.step(5)
+            out.append([eng.read(i).cpu().numpy() for i in ids])
+            eng.close()
+        for a, b in zip(*out):
+            assert np.array_equal(a, b)
