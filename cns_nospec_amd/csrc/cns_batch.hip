@@ -942,6 +942,295 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
     }
     return cmax;
 }
+#undef XMX
+#undef XMY
+#undef XMZ
+#undef XE
+#undef XP
+#undef XPF
+#undef XU
+#undef YMX
+#undef YMY
+#undef YMZ
+#undef YE
+#undef YP
+#undef YPF
+#undef YV
+#undef ZPL
+#undef ZMX
+#undef ZMY
+#undef ZMZ
+#undef ZE
+#undef ZW
+#undef ZP
+#undef ZPF
+
+// ------------------------------------------------------- staged hypterm+update tile
+// The FP32-flux hypterm + update of 32 x 8 tiles with each plane staged by
+// direct-to-LDS loads (4 bytes per lane, wrapped addresses), double-buffered, one
+// barrier per plane.  mx, my, mz, E are staged as FP64 on the tile's cross -- the own
+// rows with their x halos (10 wave-loads of 32 values per field) and the eight y-halo
+// row runs of 32 values (one wave-load each); corners are never loaded -- and p, v as
+// full tiles, u on the own rows, in the flux type.  The flux arithmetic converts each
+// FP64 value to FP32 where it reads it (the same bits as hyp_tile, which converts on
+// load), and the cell's own FP64 values for the RK update come from the staged tile,
+// so nothing is loaded twice by the lane that owns it.
+struct HypDmaGeo {
+    static constexpr int TX = 32, TY = 8, P = 40, R = 16, TILE = R * P;
+    static constexpr int NW = kBatchThreads / 64;
+    static constexpr int XS0 = 4 * P, XSN = TY * P;         // own-row strip: elements 160..479
+    static constexpr int SX64 = XSN / 32, SY64 = 8;          // FP64 position slots: strip, halo rows
+    static constexpr int S64 = SX64 + SY64;                  // 18
+    static constexpr int SF = TILE / 64, SU = XSN / 64;      // flux-type slots: full tile, strip
+    static constexpr int W64 = (S64 + NW - 1) / NW, WF = (SF + NW - 1) / NW, WU = (SU + NW - 1) / NW;
+    // buffer layout in bytes: 4 FP64 full tiles, then p and v full tiles, then the u strip
+    static constexpr int B64 = 4 * TILE * 8, BP = B64, BV = BP + TILE * 4, BU = BV + TILE * 4;
+    static constexpr int BUF = BU + XSN * 4;                 // 26880 bytes
+};
+
+// Element of FP64 position slot s (lane pair L/2) in the full-tile layout.
+__device__ __forceinline__ int hyp64_elem(int s, int l2)
+{
+    using G = HypDmaGeo;
+    if (s < G::SX64) return G::XS0 + 32 * s + l2;
+    const int hr = s - G::SX64, r = hr < 4 ? hr : hr + G::TY;  // rows 0..3, 12..15
+    return r * G::P + 4 + l2;
+}
+
+template <int stage>
+__device__ __forceinline__ unsigned long long hyp_tile_dma(BoxDev *__restrict__ boxes, int b, int x0, int y0,
+                                                           int z0, int nz, char *lds)
+{
+    using FT = float;
+    using KK = K<FT>;
+    using G = HypDmaGeo;
+    constexpr int LTX = 5, TXr = G::TX, P = G::P;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const BoxDev &B = boxes[b];
+    const TileGeo t = tile_geo<LTX>(B.e, x0, y0);
+    const int e = t.e;
+    const unsigned e2 = t.e2, e3 = t.e3, own = t.own;
+    const int row = tid >> LTX, col = tid & (TXr - 1);
+    const int me = (row + 4) * P + col + 4;  // own cell in the full-tile layout
+    const int mu = row * P + col + 4;        // own cell in the u strip
+    const double *cur = stage == 1 ? B.U : (stage == 2 ? B.S1 : B.S2);
+    double *out = stage == 1 ? B.S1 : (stage == 2 ? B.S2 : B.U);
+    const GArr<double> c_r = garr<double>((void *)cur), c_mx = c_r + e3, c_my = c_r + 2 * (size_t)e3,
+                       c_mz = c_r + 3 * (size_t)e3, c_E = c_r + 4 * (size_t)e3;
+    const GArr<double> b_0 = garr<double>(B.U), o_0 = garr<double>(out);
+    GArr<double> b_c[NC], o_c[NC];
+#pragma unroll
+    for (int l = 0; l < NC; l++) {
+        b_c[l] = b_0 + (size_t)l * e3;
+        o_c[l] = o_0 + (size_t)l * e3;
+    }
+    const GArr<FT> qi = garr<FT>(B.Q[stage - 1]), qo = garr<FT>(B.Q[stage % 3]);
+    const GArr<FT> qo_v = qo + e3, qo_w = qo + 2 * (size_t)e3, qo_p = qo + 3 * (size_t)e3,
+                   qo_t = qo + 4 * (size_t)e3;
+    const GArr<FT> din = garr<FT>(B.D);
+    const GArr<FT> d_c[4] = {din, din + e3, din + 2 * (size_t)e3, din + 3 * (size_t)e3};
+    const GArr<FT> q_u = qi, q_v = qi + e3, q_w = qi + 2 * (size_t)e3, q_p = qi + 3 * (size_t)e3;
+    const double dt = B.dt;
+    const double ra = stage == 1 ? 0.0E0 : (stage == 2 ? ThreeQuarters : OneThird);
+    const double rb = stage == 1 ? 1.0E0 : (stage == 2 ? OneQuarter : TwoThirds);
+    const double dxd[3] = {B.dx[0], B.dx[1], B.dx[2]};
+    const double pref = B.pref;
+    const FT dxi0 = (FT)B.dxi[0], dxi1 = (FT)B.dxi[1], dxi2 = (FT)B.dxi[2], prefF = (FT)pref;
+
+    // per-lane in-plane byte offsets of this wave's slots (the same for every plane)
+    unsigned o64[G::W64], of[G::WF], ou[G::WU];
+#pragma unroll
+    for (int j = 0; j < G::W64; j++) {
+        const int s = wave + j * G::NW;
+        o64[j] = s < G::S64 ? region_off(hyp64_elem(s, lane >> 1), G::TILE, P, 0, 0, x0, y0, e) * 8u +
+                                  (unsigned)(lane & 1) * 4u
+                            : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < G::WF; j++)
+        of[j] = region_off((wave + j * G::NW) * 64 + lane, G::TILE, P, 0, 0, x0, y0, e) * 4u;
+#pragma unroll
+    for (int j = 0; j < G::WU; j++)
+        ou[j] = region_off((wave + j * G::NW) * 64 + lane, G::XSN, P, 4, 0, x0, y0, e) * 4u;
+    const char GAS *const src64[4] = {(const char GAS *)c_mx.p, (const char GAS *)c_my.p,
+                                      (const char GAS *)c_mz.p, (const char GAS *)c_E.p};
+    const char GAS *const srcp = (const char GAS *)q_p.p, *const srcv = (const char GAS *)q_v.p,
+                            *const srcu = (const char GAS *)q_u.p;
+    auto issue = [&](int k, char *buf) {
+        const unsigned p8 = (unsigned)k * e2 * 8u, p4 = (unsigned)k * e2 * 4u;
+#pragma unroll
+        for (int j = 0; j < G::W64; j++) {
+            const int s = wave + j * G::NW;
+            if (s < G::S64) {
+                const int el0 = s < G::SX64 ? G::XS0 + 32 * s : hyp64_elem(s, 0);
+#pragma unroll
+                for (int f = 0; f < 4; f++)
+                    __builtin_amdgcn_global_load_lds((const void *)(src64[f] + (p8 + o64[j])),
+                                                     (lds_void *)(buf + f * G::TILE * 8 + el0 * 8), 4, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < G::WF; j++) {
+            const int s = wave + j * G::NW;
+            if (s < G::SF) {
+                __builtin_amdgcn_global_load_lds((const void *)(srcp + (p4 + of[j])),
+                                                 (lds_void *)(buf + G::BP + s * 256), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void *)(srcv + (p4 + of[j])),
+                                                 (lds_void *)(buf + G::BV + s * 256), 4, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < G::WU; j++) {
+            const int s = wave + j * G::NW;
+            if (s < G::SU)
+                __builtin_amdgcn_global_load_lds((const void *)(srcu + (p4 + ou[j])),
+                                                 (lds_void *)(buf + G::BU + s * 256), 4, 0, 0);
+        }
+    };
+    issue(z0, lds);
+
+    // z windows of w, p, mx, my, mz, E (flux type); leading values one plane ahead
+    FT ww[9], wp[9], wmx[9], wmy[9], wmz[9], wE[9];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const unsigned o = (unsigned)pmod(z0 - 4 + m, e) * e2 + own;
+        ww[m] = q_w[o]; wp[m] = q_p[o];
+        wmx[m] = (FT)c_mx[o]; wmy[m] = (FT)c_my[o]; wmz[m] = (FT)c_mz[o]; wE[m] = (FT)c_E[o];
+    }
+    FT nw, np, lmx, lmy, lmz, lE;
+    double nr;  // own rho of the next centre plane
+    {
+        const unsigned o = (unsigned)pmod(z0 + 4, e) * e2 + own;
+        nw = q_w[o]; np = q_p[o];
+        lmx = (FT)c_mx[o]; lmy = (FT)c_my[o]; lmz = (FT)c_mz[o]; lE = (FT)c_E[o];
+        nr = c_r[(unsigned)z0 * e2 + own];
+    }
+    unsigned long long cmax = 0ull;
+    for (int k = z0; k < z0 + nz; k++) {
+        const unsigned zo = (unsigned)k * e2 + own;
+        char *const bc = lds + ((k - z0) & 1) * G::BUF;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (k + 1 < z0 + nz) issue(k + 1, lds + (((k - z0) & 1) ^ 1) * G::BUF);
+        const double *s_mx = (const double *)bc, *s_my = s_mx + G::TILE, *s_mz = s_my + G::TILE,
+                     *s_E = s_mz + G::TILE;
+        const FT *s_p = (const FT *)(bc + G::BP), *s_v = (const FT *)(bc + G::BV),
+                 *s_u = (const FT *)(bc + G::BU);
+        ww[8] = nw; wp[8] = np;
+        wmx[8] = lmx; wmy[8] = lmy; wmz[8] = lmz; wE[8] = lE;
+        const double rho_c = nr;
+        if (k + 1 < z0 + nz) {  // next iteration's operands, hidden behind the compute
+            const unsigned o = (unsigned)pmod(k + 5, e) * e2 + own;
+            nw = q_w[o]; np = q_p[o];
+            lmx = (FT)c_mx[o]; lmy = (FT)c_my[o]; lmz = (FT)c_mz[o]; lE = (FT)c_E[o];
+            nr = c_r[zo + e2];
+        }
+        double rkd[4], rkb[5];
+        if (t.active) {  // this cell's RK operands
+#pragma unroll
+            for (int l = 0; l < 4; l++) rkd[l] = (double)d_c[l][zo];
+            if (stage != 1) {
+#pragma unroll
+                for (int l = 0; l < NC; l++) rkb[l] = b_c[l][zo];
+            }
+        }
+        if (t.active) {
+            FT F[5];
+#define XMX(m) ((FT)s_mx[me + (m)])
+#define XMY(m) ((FT)s_my[me + (m)])
+#define XMZ(m) ((FT)s_mz[me + (m)])
+#define XE(m) ((FT)s_E[me + (m)])
+#define XP(m) s_p[me + (m)]
+#define XPF(m) (s_p[me + (m)] + prefF)
+#define XU(m) s_u[mu + (m)]
+            F[IRHO] = -(KD1(XMX)) * dxi0;
+            F[IMX] = -(KHMOMP(XMX, XU, XP)) * dxi0;
+            F[IMY] = -(KHMOM(XMY, XU)) * dxi0;
+            F[IMZ] = -(KHMOM(XMZ, XU)) * dxi0;
+            F[IENE] = -(KHENE(XE, XU, XPF)) * dxi0;
+            FENCE();
+#define YMX(m) ((FT)s_mx[me + (m) * P])
+#define YMY(m) ((FT)s_my[me + (m) * P])
+#define YMZ(m) ((FT)s_mz[me + (m) * P])
+#define YE(m) ((FT)s_E[me + (m) * P])
+#define YP(m) s_p[me + (m) * P]
+#define YPF(m) (s_p[me + (m) * P] + prefF)
+#define YV(m) s_v[me + (m) * P]
+            F[IRHO] = F[IRHO] - (KD1(YMY)) * dxi1;
+            F[IMX] = F[IMX] - (KHMOM(YMX, YV)) * dxi1;
+            F[IMY] = F[IMY] - (KHMOMP(YMY, YV, YP)) * dxi1;
+            F[IMZ] = F[IMZ] - (KHMOM(YMZ, YV)) * dxi1;
+            F[IENE] = F[IENE] - (KHENE(YE, YV, YPF)) * dxi1;
+            FENCE();
+#define ZMX(m) wmx[4 + (m)]
+#define ZMY(m) wmy[4 + (m)]
+#define ZMZ(m) wmz[4 + (m)]
+#define ZE(m) wE[4 + (m)]
+#define ZW(m) ww[4 + (m)]
+#define ZP(m) wp[4 + (m)]
+#define ZPF(m) (wp[4 + (m)] + prefF)
+            F[IRHO] = F[IRHO] - (KD1(ZMZ)) * dxi2;
+            F[IMX] = F[IMX] - (KHMOM(ZMX, ZW)) * dxi2;
+            F[IMY] = F[IMY] - (KHMOM(ZMY, ZW)) * dxi2;
+            F[IMZ] = F[IMZ] - (KHMOMP(ZMZ, ZW, ZP)) * dxi2;
+            F[IENE] = F[IENE] - (KHENE(ZE, ZW, ZPF)) * dxi2;
+            FENCE();
+            // stage 1: out = cur + dt*(D+F) (cpu/advance.cpp:82); stages 2 and 3:
+            // out = ra*base + rb*(cur + dt*(D+F)) (:114-116, :148-150)
+            const double cur_c[5] = {rho_c, s_mx[me], s_my[me], s_mz[me], s_E[me]};
+            double o[5];
+#pragma unroll
+            for (int l = 0; l < NC; l++) {
+                const double dl = l == 0 ? 0.0E0 : rkd[l - 1];
+                const double df = dl + (double)F[l];
+                const double x = cur_c[l] + dt * df;
+                o[l] = stage == 1 ? x : ra * rkb[l] + rb * x;
+                o_c[l][zo] = o[l];
+            }
+            double u, v, w, p, T;
+            prim_cell(o[0], o[1], o[2], o[3], o[4], u, v, w, p, T);
+            qo[zo] = (FT)u;
+            qo_v[zo] = (FT)v;
+            qo_w[zo] = (FT)w;
+            qo_p[zo] = (FT)(p - pref);
+            qo_t[zo] = (FT)T;
+            if (stage == 3) {
+                const unsigned long long cb = cour_bits(o[0], u, v, w, p, dxd);
+                cmax = cmax > cb ? cmax : cb;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            ww[m] = ww[m + 1]; wp[m] = wp[m + 1];
+            wmx[m] = wmx[m + 1]; wmy[m] = wmy[m + 1]; wmz[m] = wmz[m + 1]; wE[m] = wE[m + 1];
+        }
+    }
+    return cmax;
+}
+
+#undef XMX
+#undef XMY
+#undef XMZ
+#undef XE
+#undef XP
+#undef XPF
+#undef XU
+#undef YMX
+#undef YMY
+#undef YMZ
+#undef YE
+#undef YP
+#undef YPF
+#undef YV
+#undef ZMX
+#undef ZMY
+#undef ZMZ
+#undef ZE
+#undef ZW
+#undef ZP
+#undef ZPF
+
 
 struct TileArgs {
     int b, x0, y0, z0, nz;
@@ -1028,7 +1317,25 @@ batch_hyp_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, 
     }
 }
 
-enum Pass { PASS_WZ, PASS_DIFF, PASS_DIFF_STAGED, PASS_HYP, PASS_HYP_KEEP };
+enum Pass { PASS_WZ, PASS_DIFF, PASS_DIFF_STAGED, PASS_HYP, PASS_HYP_KEEP, PASS_HYP_STAGED };
+
+// FP32 flux, 32-wide tiles: the hypterm + update tile staged by direct-to-LDS loads
+// (53.8 KB of LDS, three workgroups per CU).
+template <int stage>
+__global__ void __launch_bounds__(kBatchThreads, 2)
+batch_hyp_dma_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles)
+{
+    __shared__ double lds[2 * HypDmaGeo::BUF / 8];
+    const int ti = xcd_rank(blockIdx.x, gridDim.x);
+    if (ti >= ntiles) return;
+    const TileArgs a = read_tile(tiles, ti);
+    if (box_stopped(boxes[a.b])) return;  // uniform over the workgroup
+    const unsigned long long cm = hyp_tile_dma<stage>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, (char *)lds);
+    if (stage == 3) {
+        const unsigned long long m = block_max_u64(cm);
+        if (threadIdx.x == 0) atomicMax(&boxes[a.b].acc, m);
+    }
+}
 
 template <typename FT, int LTX>
 hipError_t stage_class(BoxDev *boxes, const TileDev *tiles, int n, int stage, Pass pass, hipStream_t s)
@@ -1038,6 +1345,11 @@ hipError_t stage_class(BoxDev *boxes, const TileDev *tiles, int n, int stage, Pa
     case PASS_DIFF: batch_diff_kernel<FT, LTX><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n, stage); break;
     case PASS_DIFF_STAGED:
         batch_diff_dma_kernel<FT, LTX><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n, stage);
+        break;
+    case PASS_HYP_STAGED:  // FP32 flux, 32-wide tiles
+        if (stage == 1) batch_hyp_dma_kernel<1><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
+        else if (stage == 2) batch_hyp_dma_kernel<2><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
+        else batch_hyp_dma_kernel<3><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
         break;
     case PASS_HYP_KEEP:  // stages 2 and 3 of the FP32 flux mode
         if (stage == 2) batch_hyp_kernel<float, LTX, 2, true><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
@@ -1076,7 +1388,7 @@ hipError_t launch_batch_finalize(BoxDev *boxes, const int *ids, int n, int count
 
 hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int start[3],
                               const int count[3], int stage, bool fp32, bool staged_diff,
-                              bool keep_centre, hipStream_t s)
+                              bool keep_centre, bool staged_hyp, hipStream_t s)
 {
     // every class's wz field (staged diffterm only), then every diffterm, then every
     // hypterm + update: each pass reads what the one before wrote on other tiles' cells
@@ -1090,10 +1402,11 @@ hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int sta
             if (n <= 0) continue;
             const TileDev *t = tiles + start[c];
             hipError_t e;
+            const Pass pc = (pass == PASS_HYP && fp32 && staged_hyp && c == 1) ? PASS_HYP_STAGED : pass;
             if (fp32)
-                e = c == 0 ? stage_class<float, 4>(boxes, t, n, stage, pass, s)
-                  : c == 1 ? stage_class<float, 5>(boxes, t, n, stage, pass, s)
-                           : stage_class<float, 6>(boxes, t, n, stage, pass, s);
+                e = c == 0 ? stage_class<float, 4>(boxes, t, n, stage, pc, s)
+                  : c == 1 ? stage_class<float, 5>(boxes, t, n, stage, pc, s)
+                           : stage_class<float, 6>(boxes, t, n, stage, pc, s);
             else
                 e = c == 0 ? stage_class<double, 4>(boxes, t, n, stage, pass, s)
                   : c == 1 ? stage_class<double, 5>(boxes, t, n, stage, pass, s)
