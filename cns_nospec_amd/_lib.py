@@ -69,6 +69,8 @@ _SIGS = {
     "cns_last_error_string": (C.c_char_p, []),
     "cns_set_stream": (_INT, [_VP]),
     "cns_get_stream": (_VP, []),
+    "cns_set_scratch": (_INT, [_VP, C.c_size_t]),
+    "cns_scratch_bytes": (C.c_size_t, [_I3, _I3, _INT]),
     "cns_set_stencil_impl": (_INT, [_INT]),
     "cns_get_stencil_impl": (_INT, []),
     "cns_fill_boundary": (_INT, [_VP, _I3, _I3]),

@@ -63,6 +63,22 @@ int geom_from_bounds(const int lo[3], const int hi[3], int ng, Geom *g)
     return CNS_OK;
 }
 
+// Caller-owned device workspace of the calling thread (cns_set_scratch): when it is
+// large enough for a call, cns_ctoprim's Courant slot and cns_diffterm's wz field live
+// there and the library allocates nothing; otherwise the per-thread buffers below are
+// allocated on first use.
+thread_local void *t_ws = nullptr;
+thread_local size_t t_ws_bytes = 0;
+thread_local int t_ws_device = -1;
+
+// The workspace's first `bytes` bytes, when the caller set one on this device that holds them.
+void *caller_workspace(size_t bytes)
+{
+    int dev = -1;
+    if (!t_ws || t_ws_bytes < bytes || hipGetDevice(&dev) != hipSuccess || dev != t_ws_device) return nullptr;
+    return t_ws;
+}
+
 // One Courant accumulator per (thread, device) for cns_ctoprim's host-courno form.
 struct TlsSlots {
     unsigned long long *acc[64] = {};
@@ -76,6 +92,10 @@ thread_local TlsSlots t_slots;
 
 int thread_acc(unsigned long long **out)
 {
+    if (void *w = caller_workspace(sizeof(unsigned long long))) {
+        *out = (unsigned long long *)w;
+        return CNS_OK;
+    }
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return fail(CNS_EINVAL, "device %d out of range", dev);
@@ -102,6 +122,10 @@ thread_local TlsScratch t_scratch;
 
 int thread_scratch(size_t count, double **out)
 {
+    if (void *w = caller_workspace(count * sizeof(double))) {
+        *out = (double *)w;
+        return CNS_OK;
+    }
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return fail(CNS_EINVAL, "device %d out of range", dev);
@@ -307,6 +331,26 @@ int cns_set_stream(void *hip_stream)
 }
 
 void *cns_get_stream(void) { return (void *)t_stream; }
+
+int cns_set_scratch(void *dev_ptr, size_t bytes)
+{
+    if (!dev_ptr && bytes) return fail(CNS_EINVAL, "NULL workspace with a nonzero size");
+    if (dev_ptr && ((uintptr_t)dev_ptr % 256) != 0)
+        return fail(CNS_EINVAL, "workspace must be 256-byte aligned");
+    int dev = -1;
+    if (dev_ptr) HIPCHK(hipGetDevice(&dev));
+    t_ws = dev_ptr;
+    t_ws_bytes = dev_ptr ? bytes : 0;
+    t_ws_device = dev;
+    return CNS_OK;
+}
+
+size_t cns_scratch_bytes(const int lo[3], const int hi[3], int ng)
+{
+    Geom g;
+    if (geom_from_bounds(lo, hi, ng, &g)) return 0;
+    return (size_t)g.cg * sizeof(double);  // the wz field; the Courant slot fits in it
+}
 
 int cns_set_stencil_impl(int impl)
 {

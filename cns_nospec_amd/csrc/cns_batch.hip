@@ -1273,10 +1273,13 @@ batch_wz_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, i
     wz_tile<FT, LTX>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, stage);
 }
 
-// Staged diffterm: LDS-limited to 4 workgroups per CU (FP32, 32-wide tiles: 35 KB and
-// at most 128 VGPRs), 2 in FP64.
+// Staged diffterm: as many workgroups per CU as its LDS allows (FP32: 32-35 KB for 16-
+// and 32-wide tiles, four, with at most 128 VGPRs; 48 KB for 64-wide, three; FP64 twice
+// that: two, and one).
 template <typename FT, int LTX>
-__global__ void __launch_bounds__(kBatchThreads, sizeof(FT) == 4 ? 4 : 2)
+__global__ void __launch_bounds__(kBatchThreads, (160 * 1024) / (DmaGeo<FT, LTX>::LDS_ELEMS * (int)sizeof(FT)) > 4
+                                                     ? 4
+                                                     : (160 * 1024) / (DmaGeo<FT, LTX>::LDS_ELEMS * (int)sizeof(FT)))
 batch_diff_dma_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles, int stage)
 {
     __shared__ FT lds[DmaGeo<FT, LTX>::LDS_ELEMS];

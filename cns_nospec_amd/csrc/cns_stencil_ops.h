@@ -28,34 +28,27 @@ struct K {
     static constexpr T Two = T(2.0E0);
 };
 
-// acc + a*b.  In the flux type float it is one fused multiply-add (one rounding, one
-// VALU instruction); in double it is the reference's separate product and sum, so the
-// FP64 path keeps the reference's bits (the library is compiled without contraction).
-__device__ __forceinline__ float mac(float acc, float a, float b) { return __builtin_fmaf(a, b, acc); }
-__device__ __forceinline__ double mac(double acc, double a, double b) { return acc + a * b; }
-
 }  // namespace cns
 
-// The operators below are chains of mac() in the reference's left-to-right order.  In
-// double every step rounds exactly as the reference's expression does (x - y == -y + x
-// in IEEE arithmetic), so the FP64 results are bit-identical to the unfused forms.
-
 // First difference sum (before *dxinv).
-#define KD1(F)                                                                                 \
-    cns::mac(cns::mac(cns::mac(KK::ALP * (F(1) - F(-1)), KK::BET, F(2) - F(-2)), KK::GAM,     \
-                      F(3) - F(-3)),                                                          \
-             KK::DEL, F(4) - F(-4))
+#define KD1(F)                                                                      \
+    (KK::ALP * (F(1) - F(-1)) + KK::BET * (F(2) - F(-2)) + KK::GAM * (F(3) - F(-3)) \
+     + KK::DEL * (F(4) - F(-4)))
 // Second difference sum (before *dxinv^2).
-#define KD2(F)                                                                                 \
-    cns::mac(cns::mac(cns::mac(cns::mac(KK::CENTER * F(0), KK::OFF1, F(1) + F(-1)), KK::OFF2, \
-                               F(2) + F(-2)),                                                 \
-                      KK::OFF3, F(3) + F(-3)),                                                \
-             KK::OFF4, F(4) + F(-4))
-// C(m)*UN(m) - C(-m)*UN(-m)
-#define KPD(C, UN, m) cns::mac(-(C(-(m)) * UN(-(m))), C(m), UN(m))
+#define KD2(F)                                                                              \
+    (KK::CENTER * F(0) + KK::OFF1 * (F(1) + F(-1)) + KK::OFF2 * (F(2) + F(-2))            \
+     + KK::OFF3 * (F(3) + F(-3)) + KK::OFF4 * (F(4) + F(-4)))
 // hypterm: tangential momentum flux sum, normal momentum (with pressure), energy.
-#define KHSUM(T)                                                                               \
-    cns::mac(cns::mac(cns::mac(KK::ALP * (T(1)), KK::BET, T(2)), KK::GAM, T(3)), KK::DEL, T(4))
-#define KHMOM(C, UN) KHSUM([&](int m_) { return KPD(C, UN, m_); })
-#define KHMOMP(C, UN, P) KHSUM([&](int m_) { return KPD(C, UN, m_) + (P(m_) - P(-m_)); })
-#define KHENE(C, UN, P) KHSUM([&](int m_) { return KPD(C, UN, m_) + KPD(P, UN, m_); })
+#define KHMOM(C, UN)                                                                 \
+    (KK::ALP * (C(1) * UN(1) - C(-1) * UN(-1)) + KK::BET * (C(2) * UN(2) - C(-2) * UN(-2)) \
+     + KK::GAM * (C(3) * UN(3) - C(-3) * UN(-3)) + KK::DEL * (C(4) * UN(4) - C(-4) * UN(-4)))
+#define KHMOMP(C, UN, P)                                                              \
+    (KK::ALP * (C(1) * UN(1) - C(-1) * UN(-1) + (P(1) - P(-1)))                       \
+     + KK::BET * (C(2) * UN(2) - C(-2) * UN(-2) + (P(2) - P(-2)))                     \
+     + KK::GAM * (C(3) * UN(3) - C(-3) * UN(-3) + (P(3) - P(-3)))                     \
+     + KK::DEL * (C(4) * UN(4) - C(-4) * UN(-4) + (P(4) - P(-4))))
+#define KHENE(C, UN, P)                                                               \
+    (KK::ALP * (C(1) * UN(1) - C(-1) * UN(-1) + (P(1) * UN(1) - P(-1) * UN(-1)))      \
+     + KK::BET * (C(2) * UN(2) - C(-2) * UN(-2) + (P(2) * UN(2) - P(-2) * UN(-2)))    \
+     + KK::GAM * (C(3) * UN(3) - C(-3) * UN(-3) + (P(3) * UN(3) - P(-3) * UN(-3)))    \
+     + KK::DEL * (C(4) * UN(4) - C(-4) * UN(-4) + (P(4) * UN(4) - P(-4) * UN(-4))))

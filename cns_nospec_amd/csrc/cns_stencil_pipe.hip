@@ -831,9 +831,15 @@ diffterm_pipe_kernel(Geom g, const double *__restrict__ q, const double *__restr
 // runs down to 4 planes until a launch has 256 workgroups (64^3 step 0.60 -> 0.40 ms,
 // 32^3 0.55 -> 0.35 ms; at 128^3, with 512 workgroups at runs of 16, shorter runs are
 // slower).
+// CNS_PIPE_KZ in the environment caps the run length (for measurements).
 int pipe_kz(const Geom &g, int tiles_xy, int nbox = 1)
 {
     int kz = g.n2;
+    static const int cap = [] {
+        const char *e = getenv("CNS_PIPE_KZ");
+        return e ? atoi(e) : 0;
+    }();
+    if (cap >= 4 && kz > cap) kz = cap;
     auto blocks = [&](int z) { return (int64_t)tiles_xy * nbox * ((g.n2 + z - 1) / z); };
     while (kz > 16 && blocks(kz) < 2048) kz = (kz + 1) / 2;
     while (kz > 4 && blocks(kz) < 256) kz = (kz + 1) / 2;

@@ -156,7 +156,7 @@ class BatchServer:
 
     def __init__(self, hbm_budget: int, fp32: bool = True, max_boxes: int = 4096,
                  max_chunk: int = 16, keep_states=(), progress_every: int = 0,
-                 max_resident_cells: int = 128 << 20):
+                 max_resident_cells: int = 128 << 20, prepare_ahead: int = 16):
         from .batch import BatchEngine
         self.engine = BatchEngine(hbm_budget, fp32=fp32, max_boxes=max_boxes)
         self.max_chunk = max_chunk
@@ -168,20 +168,35 @@ class BatchServer:
         self.max_resident_cells = max_resident_cells
         self.keep = set(keep_states)
         self.progress_every = progress_every
+        # Initial states of up to this many waiting requests are built while a chunk runs
+        # (their host work then overlaps the GPU's), so admission itself is a copy.
+        self.prepare_ahead = prepare_ahead
+        # per chunk: (stream time at launch, steps, resident boxes, resident cells,
+        # seconds from launch to the end of its synchronising query)
+        self.chunks = []
 
     def run(self, requests):
         import torch
         eng = self.engine
         clock = _Clock()
-        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        # FIFO by arrival; requests that arrive together are taken longest first (every
+        # resident box advances one step per engine step, so a box's steps set when it
+        # finishes, and the last boxes admitted then run the shortest jobs: no tail of a
+        # few long boxes running alone on an idle GPU)
+        pending = deque(sorted(requests, key=lambda r: (r.arrival, -r.steps, r.rid)))
         waiting = deque()
+        prepared = {}  # request id -> initial state, for requests at the head of `waiting`
         active = {}  # box id -> (request, steps left, result)
         results = {}
         resident = 0  # cells of the active boxes
-        while pending or waiting or active:
+
+        def arrivals():
             now = clock.now()
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
+
+        while pending or waiting or active:
+            arrivals()
             # admission: FIFO while the budget has room; each box stops by itself on the
             # device once it has taken its requested steps
             while waiting:
@@ -189,10 +204,12 @@ class BatchServer:
                 if (self.max_resident_cells and active
                         and resident + req.edge ** 3 > self.max_resident_cells):
                     break
+                u0 = prepared.get(req.rid)
                 try:
-                    bid = eng.add(request_state(req).reshape(-1), req.edge)
+                    bid = eng.add(u0 if u0 is not None else request_state(req).reshape(-1), req.edge)
                 except _lib.CnsBudgetError:
                     break
+                prepared.pop(req.rid, None)
                 eng.set_target(bid, req.steps)
                 waiting.popleft()
                 res = Result(req.rid, req.edge, req.steps, req.arrival, admitted=clock.now())
@@ -210,8 +227,16 @@ class BatchServer:
             # chunk need not end at the first box's end (that would cost a sync and a
             # work-list rebuild per finishing box)
             k = min(self.max_chunk, max(v[1] for v in active.values()))
+            t_launch = clock.now()
             eng.step(k)
+            # while the chunk runs: the next admissions' initial states (their kernels queue
+            # behind the chunk on the same stream; the host work overlaps it)
+            arrivals()
+            for req in list(waiting)[:self.prepare_ahead]:
+                if req.rid not in prepared:
+                    prepared[req.rid] = request_state(req).reshape(-1)
             q = eng.query(list(active))  # synchronises
+            self.chunks.append((t_launch, k, len(active), resident, clock.now() - t_launch))
             for bid, v in active.items():
                 v[1] = v[0].steps - q[bid]["steps"]
             finished = [bid for bid, v in active.items() if v[1] <= 0 or q[bid]["err"]]

@@ -25,6 +25,8 @@
 #ifndef CNS_H
 #define CNS_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -43,6 +45,20 @@ const char *cns_last_error_string(void);
  * The reference always uses the default stream (gpu/advance.cu:107-203). */
 int cns_set_stream(void *hip_stream);
 void *cns_get_stream(void);
+
+/* Device workspace for the free-standing per-kernel calls of the calling thread:
+ * cns_ctoprim with a non-NULL courno needs one 8-byte slot (its Courant accumulator),
+ * cns_diffterm one ghosted component of doubles (the wz field of the tiled kernel);
+ * cns_scratch_bytes() gives the size for a box (0 for bad bounds).  With a workspace of
+ * at least that size set on the current device (dev_ptr 256-byte aligned, owned and
+ * freed by the caller, not used by it while the calls that read it run), those calls
+ * allocate nothing.  Without one they allocate a per-thread, per-device buffer on first
+ * use (grown when a larger box needs it) and keep it until the thread exits.
+ * cns_set_scratch(NULL, 0) removes it.  cns_advance and the other context calls use the
+ * context's own buffers.  No counterpart in the reference, whose per-call temporaries
+ * are allocated by the caller at start-up (gpu/util.cu:171-175). */
+int cns_set_scratch(void *dev_ptr, size_t bytes);
+size_t cns_scratch_bytes(const int lo[3], const int hi[3], int ng);
 
 /* Which kernels cns_hypterm, cns_diffterm and the step use (process-wide):
  * CNS_IMPL_TILED (default) -- z-marching LDS-tiled kernels;

@@ -66,3 +66,19 @@ def test_seeded_state_is_deterministic_and_physical():
     U = sinusoid_state(lay, 5)
     assert np.all(lay.gview(U, NC)[0] > 0)
     assert box_dx(16) == (0.0125, 0.0125, 0.0125)
+
+
+def test_scratch_size_and_argument_checks():
+    """cns_scratch_bytes is one ghosted component of doubles (the wz field; the Courant
+    slot fits in it), 0 for bounds the library rejects; cns_set_scratch refuses a NULL
+    buffer with a size and a misaligned one before touching the device, and NULL, 0
+    (no workspace) is always accepted."""
+    lib = _lib.load()
+    I3 = ctypes.c_int * 3
+    lay = BoxLayout((20, 12, 17))
+    assert lib.cns_scratch_bytes(I3(*lay.lo), I3(*lay.hi), 4) == lay.cg * 8
+    assert lib.cns_scratch_bytes(I3(*lay.lo), I3(*lay.hi), 3) == 0
+    assert lib.cns_set_scratch(None, 0) == _lib.CNS_OK
+    assert lib.cns_set_scratch(None, 64) == _lib.CNS_EINVAL
+    assert lib.cns_set_scratch(ctypes.c_void_p(0x1008), 4096) == _lib.CNS_EINVAL
+    assert b"aligned" in lib.cns_last_error_string()

@@ -236,6 +236,36 @@ def test_step_compositions_agree(device, monkeypatch, n):
             assert np.array_equal(a, b), f"{name}: {what}"
 
 
+def test_caller_workspace_gives_the_same_bits(device):
+    """With a caller-owned workspace set (cns_set_scratch, sized by cns_scratch_bytes),
+    cns_ctoprim's Courant reduction and cns_diffterm run in it and give the bits they
+    give with the library's own per-thread buffers."""
+    import ctypes as C
+    import torch
+    lay = BoxLayout((40, 24, 33))
+    dx = box_dx(lay.n)
+    U = dev(sinusoid_state(lay, 77, mach=0.25, kmax=3), device)
+    lib = _lib.load()
+    I3 = C.c_int * 3
+    nbytes = lib.cns_scratch_bytes(I3(*lay.lo), I3(*lay.hi), lay.ng)
+    assert nbytes == lay.cg * 8
+    ws = torch.empty(nbytes // 8 + 32, dtype=torch.float64, device=device)
+    base = (ws.data_ptr() + 255) // 256 * 256
+    out = []
+    for use_ws in (False, True):
+        _lib.check(lib.cns_set_scratch(C.c_void_p(base if use_ws else None), nbytes if use_ws else 0))
+        Q = dev(lay.new_ghosted(NQ), device)
+        D = dev(lay.new_interior(NC, np.nan), device)
+        c = ops.ctoprim(lay.lo, lay.hi, U, Q, dx, lay.ng, 1.0e-50)
+        ops.diffterm(lay.lo, lay.hi, lay.ng, dx, Q, D, ETA, ALAM)
+        out.append((c, host(Q), host(D)))
+    _lib.check(lib.cns_set_scratch(None, 0))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(lay.iview(out[0][2], NC), lay.iview(out[1][2], NC))
+    assert not np.isnan(lay.iview(out[1][2], NC)).any()
+
+
 def test_nonfinite_state_is_an_error(device):
     lay = BoxLayout.cube(16)
     U = sinusoid_state(lay, 3)

@@ -75,8 +75,9 @@ class _Flat:
 
 
 def test_batch_server_admission_targets_and_cap(monkeypatch):
-    """FIFO admission under the HBM budget and the resident-cell cap, each box run to
-    exactly its own step target, every request served once."""
+    """FIFO admission under the HBM budget and the resident-cell cap (requests that arrive
+    together taken longest first), each box run to exactly its own step target, every
+    request served once."""
     from cns_nospec_amd import batch, serve
     monkeypatch.setattr(batch, "BatchEngine", _FakeEngine)
     monkeypatch.setattr(serve, "request_state", lambda req: _Flat())
@@ -87,8 +88,26 @@ def test_batch_server_admission_targets_and_cap(monkeypatch):
     out, summ = srv.run(reqs)
     assert summ["requests"] == 40 and summ["failed"] == 0
     assert [r.rid for r in out] == list(range(40))
+    order = sorted(reqs, key=lambda r: (r.arrival, -r.steps, r.rid))  # all arrive at t = 0
     assert [e[1] for e in _FakeEngine.log] == list(range(40))  # one admission each
-    assert [e[2] for e in _FakeEngine.log] == [r.edge for r in reqs]  # in arrival (FIFO) order
+    assert [e[2] for e in _FakeEngine.log] == [r.edge for r in order]
     assert cap // 2 < eng.resident_peak <= cap  # the cap binds, and holds
     assert all(r.done >= r.admitted >= r.arrival for r in out)
     assert summ["cell_steps"] == sum(r.edge ** 3 * r.steps for r in reqs)
+
+
+def test_batch_server_fifo_across_arrival_times(monkeypatch):
+    """Distinct arrival times keep first-come-first-served order whatever the step counts;
+    only requests that arrive together are reordered (longest first)."""
+    from cns_nospec_amd import batch, serve
+    monkeypatch.setattr(batch, "BatchEngine", _FakeEngine)
+    monkeypatch.setattr(serve, "request_state", lambda req: _Flat())
+    reqs = [serve.Request(i, 0.01 * (i // 2), 20, 10 + 7 * (i % 2) + i, i, 0.1) for i in range(12)]
+    srv = serve.BatchServer(10 ** 9, fp32=True, max_chunk=3, max_resident_cells=3 * 20 ** 3)
+    out, summ = srv.run(reqs)
+    assert summ["failed"] == 0
+    admitted = [e[1] for e in _FakeEngine.log]
+    edges_order = [reqs[i] for i in sorted(range(12), key=lambda i: (reqs[i].arrival, -reqs[i].steps, i))]
+    assert len(admitted) == 12
+    assert [r.arrival for r in sorted(out, key=lambda r: r.admitted)] == sorted(r.arrival for r in reqs)
+    assert [r.rid for r in edges_order][:2] == [1, 0]  # the pair arriving first: longer one first
