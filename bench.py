@@ -466,7 +466,11 @@ def served_secondary(nreq: int, seed: int = 2024, poisson_nreq: int = 200):
                                                   "read U1 and U, write U2; read U2 and U, write U)",
                          "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(ach / HBM_PEAK_GBS, 4),
-                         "traffic_bytes_per_cell_step": measured},
+                         "traffic_bytes_per_cell_step": measured,
+                         "traffic_vs_minimum": round(measured / BATCHED_MIN_BYTES_PER_CELL_STEP, 2)
+                         if measured else None,
+                         "traffic_GBps_at_served_rate": round(measured * bat["cell_steps_per_s"] / 1e9, 1)
+                         if measured else None},
             "fp32_flux_error_vs_fp64": errs,
             "error_sample": f"requests {list(keep)} of the burst run, final state (50-500 steps)"}
 
