@@ -966,44 +966,51 @@ __device__ __forceinline__ unsigned long long hyp_tile(BoxDev *__restrict__ boxe
 #undef ZPF
 
 // ------------------------------------------------------- staged hypterm+update tile
-// The FP32-flux hypterm + update of 32 x 8 tiles with each plane staged by
-// direct-to-LDS loads (4 bytes per lane, wrapped addresses), double-buffered, one
-// barrier per plane.  mx, my, mz, E are staged as FP64 on the tile's cross -- the own
-// rows with their x halos (10 wave-loads of 32 values per field) and the eight y-halo
-// row runs of 32 values (one wave-load each); corners are never loaded -- and p, v as
-// full tiles, u on the own rows, in the flux type.  The flux arithmetic converts each
-// FP64 value to FP32 where it reads it (the same bits as hyp_tile, which converts on
-// load), and the cell's own FP64 values for the RK update come from the staged tile,
-// so nothing is loaded twice by the lane that owns it.
-struct HypDmaGeo {
+// The hypterm + update of 32 x 8 tiles with each plane staged by direct-to-LDS loads
+// (4 bytes per lane, wrapped addresses), double-buffered, one barrier per plane.  mx,
+// my, mz, E are staged as FP64 on the tile's cross -- the own rows with their x halos
+// (10 wave-loads of 32 values per field) and the eight y-halo row runs of 32 values
+// (one wave-load each); corners are never loaded -- and p, v as full tiles, u on the
+// own rows, in the flux type (an FP64 value is two 4-byte lanes).  In the FP32-flux
+// mode the flux arithmetic converts each FP64 value to FP32 where it reads it (the same
+// bits as hyp_tile, which converts on load); in the FP64 mode every value is used as
+// loaded.  The cell's own FP64 values for the RK update come from the staged tile, so
+// nothing is loaded twice by the lane that owns it.
+struct HypDmaBase {
     static constexpr int TX = 32, TY = 8, P = 40, R = 16, TILE = R * P;
     static constexpr int NW = kBatchThreads / 64;
     static constexpr int XS0 = 4 * P, XSN = TY * P;         // own-row strip: elements 160..479
     static constexpr int SX64 = XSN / 32, SY64 = 8;          // FP64 position slots: strip, halo rows
     static constexpr int S64 = SX64 + SY64;                  // 18
-    static constexpr int SF = TILE / 64, SU = XSN / 64;      // flux-type slots: full tile, strip
-    static constexpr int W64 = (S64 + NW - 1) / NW, WF = (SF + NW - 1) / NW, WU = (SU + NW - 1) / NW;
+    static constexpr int W64 = (S64 + NW - 1) / NW;
+    static constexpr int B64 = 4 * TILE * 8;                 // bytes of the 4 FP64 tiles
+};
+template <typename FT>
+struct HypDmaGeo : HypDmaBase {
+    static constexpr int FB = (int)sizeof(FT);
+    static constexpr int EPS = 256 / FB;                     // flux-type elements per wave-load
+    static constexpr int SF = TILE / EPS, SU = XSN / EPS;    // flux-type slots: full tile, strip
+    static constexpr int WF = (SF + NW - 1) / NW, WU = (SU + NW - 1) / NW;
     // buffer layout in bytes: 4 FP64 full tiles, then p and v full tiles, then the u strip
-    static constexpr int B64 = 4 * TILE * 8, BP = B64, BV = BP + TILE * 4, BU = BV + TILE * 4;
-    static constexpr int BUF = BU + XSN * 4;                 // 26880 bytes
+    static constexpr int BP = B64, BV = BP + TILE * FB, BU = BV + TILE * FB;
+    static constexpr int BUF = BU + XSN * FB;                // 26880 (FP32) / 33280 (FP64) bytes
 };
 
 // Element of FP64 position slot s (lane pair L/2) in the full-tile layout.
 __device__ __forceinline__ int hyp64_elem(int s, int l2)
 {
-    using G = HypDmaGeo;
+    using G = HypDmaBase;
     if (s < G::SX64) return G::XS0 + 32 * s + l2;
     const int hr = s - G::SX64, r = hr < 4 ? hr : hr + G::TY;  // rows 0..3, 12..15
     return r * G::P + 4 + l2;
 }
 
-template <int stage>
+template <typename FT, int stage>
 __device__ __forceinline__ unsigned long long hyp_tile_dma(BoxDev *__restrict__ boxes, int b, int x0, int y0,
                                                            int z0, int nz, char *lds)
 {
-    using FT = float;
     using KK = K<FT>;
-    using G = HypDmaGeo;
+    using G = HypDmaGeo<FT>;
     constexpr int LTX = 5, TXr = G::TX, P = G::P;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1047,18 +1054,21 @@ __device__ __forceinline__ unsigned long long hyp_tile_dma(BoxDev *__restrict__ 
                                   (unsigned)(lane & 1) * 4u
                             : 0u;
     }
+    // flux-type slots: element lane / (FB / 4) of the slot, and for FP64 the lane's half
+    constexpr int LPE = G::FB / 4;
+    const unsigned half = (unsigned)(lane & (LPE - 1)) * 4u;
 #pragma unroll
     for (int j = 0; j < G::WF; j++)
-        of[j] = region_off((wave + j * G::NW) * 64 + lane, G::TILE, P, 0, 0, x0, y0, e) * 4u;
+        of[j] = region_off((wave + j * G::NW) * G::EPS + lane / LPE, G::TILE, P, 0, 0, x0, y0, e) * G::FB + half;
 #pragma unroll
     for (int j = 0; j < G::WU; j++)
-        ou[j] = region_off((wave + j * G::NW) * 64 + lane, G::XSN, P, 4, 0, x0, y0, e) * 4u;
+        ou[j] = region_off((wave + j * G::NW) * G::EPS + lane / LPE, G::XSN, P, 4, 0, x0, y0, e) * G::FB + half;
     const char GAS *const src64[4] = {(const char GAS *)c_mx.p, (const char GAS *)c_my.p,
                                       (const char GAS *)c_mz.p, (const char GAS *)c_E.p};
     const char GAS *const srcp = (const char GAS *)q_p.p, *const srcv = (const char GAS *)q_v.p,
                             *const srcu = (const char GAS *)q_u.p;
     auto issue = [&](int k, char *buf) {
-        const unsigned p8 = (unsigned)k * e2 * 8u, p4 = (unsigned)k * e2 * 4u;
+        const unsigned p8 = (unsigned)k * e2 * 8u, p4 = (unsigned)k * e2 * (unsigned)G::FB;
 #pragma unroll
         for (int j = 0; j < G::W64; j++) {
             const int s = wave + j * G::NW;
@@ -1322,18 +1332,18 @@ batch_hyp_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, 
 
 enum Pass { PASS_WZ, PASS_DIFF, PASS_DIFF_STAGED, PASS_HYP, PASS_HYP_KEEP, PASS_HYP_STAGED };
 
-// FP32 flux, 32-wide tiles: the hypterm + update tile staged by direct-to-LDS loads
-// (53.8 KB of LDS, three workgroups per CU).
-template <int stage>
+// 32-wide tiles: the hypterm + update tile staged by direct-to-LDS loads (FP32 flux:
+// 53.8 KB of LDS, three workgroups per CU; FP64: 66.6 KB, two).
+template <typename FT, int stage>
 __global__ void __launch_bounds__(kBatchThreads, 2)
 batch_hyp_dma_kernel(BoxDev *__restrict__ boxes, const TileDev *__restrict__ tiles, int ntiles)
 {
-    __shared__ double lds[2 * HypDmaGeo::BUF / 8];
+    __shared__ double lds[2 * HypDmaGeo<FT>::BUF / 8];
     const int ti = xcd_rank(blockIdx.x, gridDim.x);
     if (ti >= ntiles) return;
     const TileArgs a = read_tile(tiles, ti);
     if (box_stopped(boxes[a.b])) return;  // uniform over the workgroup
-    const unsigned long long cm = hyp_tile_dma<stage>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, (char *)lds);
+    const unsigned long long cm = hyp_tile_dma<FT, stage>(boxes, a.b, a.x0, a.y0, a.z0, a.nz, (char *)lds);
     if (stage == 3) {
         const unsigned long long m = block_max_u64(cm);
         if (threadIdx.x == 0) atomicMax(&boxes[a.b].acc, m);
@@ -1349,10 +1359,11 @@ hipError_t stage_class(BoxDev *boxes, const TileDev *tiles, int n, int stage, Pa
     case PASS_DIFF_STAGED:
         batch_diff_dma_kernel<FT, LTX><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n, stage);
         break;
-    case PASS_HYP_STAGED:  // FP32 flux, 32-wide tiles
-        if (stage == 1) batch_hyp_dma_kernel<1><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
-        else if (stage == 2) batch_hyp_dma_kernel<2><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
-        else batch_hyp_dma_kernel<3><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
+    case PASS_HYP_STAGED:  // 32-wide tiles only
+        if (LTX != 5) return hipErrorInvalidValue;
+        if (stage == 1) batch_hyp_dma_kernel<FT, 1><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
+        else if (stage == 2) batch_hyp_dma_kernel<FT, 2><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
+        else batch_hyp_dma_kernel<FT, 3><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
         break;
     case PASS_HYP_KEEP:  // stages 2 and 3 of the FP32 flux mode
         if (stage == 2) batch_hyp_kernel<float, LTX, 2, true><<<n, kBatchThreads, 0, s>>>(boxes, tiles, n);
@@ -1391,7 +1402,7 @@ hipError_t launch_batch_finalize(BoxDev *boxes, const int *ids, int n, int count
 
 hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int start[3],
                               const int count[3], int stage, bool fp32, bool staged_diff,
-                              bool keep_centre, bool staged_hyp, hipStream_t s)
+                              bool keep_centre, bool staged_hyp, bool staged_hyp64, hipStream_t s)
 {
     // every class's wz field (staged diffterm only), then every diffterm, then every
     // hypterm + update: each pass reads what the one before wrote on other tiles' cells
@@ -1405,15 +1416,16 @@ hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int sta
             if (n <= 0) continue;
             const TileDev *t = tiles + start[c];
             hipError_t e;
-            const Pass pc = (pass == PASS_HYP && fp32 && staged_hyp && c == 1) ? PASS_HYP_STAGED : pass;
+            const bool staged = fp32 ? staged_hyp : staged_hyp64;
+            const Pass pc = (pass == PASS_HYP && staged && c == 1) ? PASS_HYP_STAGED : pass;
             if (fp32)
                 e = c == 0 ? stage_class<float, 4>(boxes, t, n, stage, pc, s)
                   : c == 1 ? stage_class<float, 5>(boxes, t, n, stage, pc, s)
                            : stage_class<float, 6>(boxes, t, n, stage, pc, s);
             else
-                e = c == 0 ? stage_class<double, 4>(boxes, t, n, stage, pass, s)
-                  : c == 1 ? stage_class<double, 5>(boxes, t, n, stage, pass, s)
-                           : stage_class<double, 6>(boxes, t, n, stage, pass, s);
+                e = c == 0 ? stage_class<double, 4>(boxes, t, n, stage, pc, s)
+                  : c == 1 ? stage_class<double, 5>(boxes, t, n, stage, pc, s)
+                           : stage_class<double, 6>(boxes, t, n, stage, pc, s);
             if (e != hipSuccess) return e;
         }
     }

@@ -52,10 +52,11 @@ hipError_t launch_batch_finalize(BoxDev *boxes, const int *box_ids, int nbox, in
 // hypterm + update launch, of one workgroup per tile.  staged_diff selects the
 // diffterm whose planes are staged by direct-to-LDS loads, preceded by a wz-field
 // launch per class; keep_centre the FP32 hypterm + update of stages 2 and 3 that keeps
-// its window's FP64 leading values for the centre plane; staged_hyp the FP32 hypterm +
-// update of 32-wide tiles whose planes are staged by direct-to-LDS loads.
+// its window's FP64 leading values for the centre plane; staged_hyp (FP32 flux) and
+// staged_hyp64 (FP64) the hypterm + update of 32-wide tiles whose planes are staged by
+// direct-to-LDS loads.
 hipError_t launch_batch_stage(BoxDev *boxes, const TileDev *tiles, const int start[3],
                               const int count[3], int stage, bool fp32, bool staged_diff,
-                              bool keep_centre, bool staged_hyp, hipStream_t s);
+                              bool keep_centre, bool staged_hyp, bool staged_hyp64, hipStream_t s);
 
 }  // namespace cns

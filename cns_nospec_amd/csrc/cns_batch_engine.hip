@@ -121,9 +121,11 @@ struct cns_batch {
     // which cost more than the 32 bytes per cell it saves: 4.66 vs 4.73 G cell-steps/s
     // on the request mix).
     bool keep_centre = false;
-    // FP32 flux, 32-wide tiles: the hypterm + update tile staged by direct-to-LDS loads
-    // (CNS_BATCH_HYP=0 selects the tile that gathers its halos through registers).
+    // 32-wide tiles: the hypterm + update tile staged by direct-to-LDS loads, FP32 flux
+    // and FP64 (CNS_BATCH_HYP=0, CNS_BATCH_HYP64=0 select the tile that gathers its halos
+    // through registers; the same bits either way).
     bool staged_hyp = true;
+    bool staged_hyp64 = true;
     bool dirty = true;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -215,7 +217,7 @@ int rebuild(cns_batch *b, hipStream_t s)
         hipError_t e = hipSuccess;
         for (int stage = 1; stage <= 3 && e == hipSuccess; stage++)
             e = launch_batch_stage(b->d_boxes, b->d_tiles, b->cls_start, b->cls_count, stage, b->fp32,
-                                   b->staged_diff, b->keep_centre, b->staged_hyp,
+                                   b->staged_diff, b->keep_centre, b->staged_hyp, b->staged_hyp64,
                                    b->cap_stream);
         if (e == hipSuccess) e = launch_batch_finalize(b->d_boxes, b->d_ids, b->nresident, 1, b->cap_stream);
         hipGraph_t g = nullptr;
@@ -258,6 +260,7 @@ int cns_batch_create(cns_batch **out, size_t hbm_budget, int flux_fp32, int max_
     if (const char *env = getenv("CNS_BATCH_DIFF")) b->staged_diff = atoi(env) != 0;
     if (const char *env = getenv("CNS_BATCH_KEEP")) b->keep_centre = atoi(env) != 0;
     if (const char *env = getenv("CNS_BATCH_HYP")) b->staged_hyp = atoi(env) != 0;
+    if (const char *env = getenv("CNS_BATCH_HYP64")) b->staged_hyp64 = atoi(env) != 0;
     int rc = CNS_OK;
     if (hipGetDevice(&b->device) != hipSuccess) rc = fail(CNS_EHIP, "hipGetDevice failed");
     if (!rc && hbm_budget && hipMalloc((void **)&b->pool_mem, hbm_budget) != hipSuccess) {

@@ -198,10 +198,10 @@ def test_engine_variants_agree(device, monkeypatch, fp32):
     """The engine's kernel variants compute the same expressions on the same values, so
     they give the same bits, for every tile width and partial tiles: the diffterm tile
     whose planes are staged by direct-to-LDS loads (with a wz field) against the one that
-    gathers its halos through registers, the FP32 hypterm tile staged by direct-to-LDS
-    loads against the one that gathers its halos, and the FP32 hypterm tile that keeps
-    its z window's FP64 leading values for the centre plane against the one that loads
-    them again."""
+    gathers its halos through registers, the hypterm tile staged by direct-to-LDS loads
+    (FP32 flux and FP64) against the one that gathers its halos, and the FP32 hypterm
+    tile that keeps its z window's FP64 leading values for the centre plane against the
+    one that loads them again."""
     edges = (16, 20, 24, 40)
     init = {e: compact_sinusoid(e, 90 + e, mach=0.25, kmax=3) for e in edges}
     tiles = {16: (16, 5), 20: (32, 7), 24: (64, 24), 40: (32, 16)}
@@ -210,6 +210,7 @@ def test_engine_variants_agree(device, monkeypatch, fp32):
         monkeypatch.setenv("CNS_BATCH_DIFF", diff)
         monkeypatch.setenv("CNS_BATCH_KEEP", keep)
         monkeypatch.setenv("CNS_BATCH_HYP", hyp)
+        monkeypatch.setenv("CNS_BATCH_HYP64", hyp)
         eng = BatchEngine(2 << 30, fp32=fp32, max_boxes=8, tuning=tiles)
         ids = {e: eng.add(init[e].reshape(-1), e) for e in edges}
         eng.step(6)
