@@ -56,13 +56,14 @@ def test_driver_output_matches_reference_program(tmp_path, device, case):
 
 
 def test_driver_named_case_and_step_override(tmp_path, device, orc):
-    """<name>_general_input / <name>_multistep_input and -n; a 64^3 box against the
-    oracle, byte for byte."""
+    """<name>_general_input / <name>_multistep_input and -n; a 64^3 box of seeded
+    sinusoids (a named case of the reference's file format, not its cell64 CGS deck,
+    which test_gpu_parity covers) against the oracle, byte for byte."""
     lay = BoxLayout.cube(64)
     U = sinusoid_state(lay, 11, kmax=2)
     cfg = refio.make_config(64, box_dx(64), CFL, ETA, ALAM, 100)
-    refio.write_testcase(str(tmp_path / "testcases"), cfg, U, name="cell64")
-    r = _run(tmp_path, name="cell64", nsteps=4, quiet=True)
+    refio.write_testcase(str(tmp_path / "testcases"), cfg, U, name="sin64")
+    r = _run(tmp_path, name="sin64", nsteps=4, quiet=True)
     assert refio.step_lines(r.stdout) == []
     Unew, Q = lay.new_ghosted(NC), lay.new_ghosted(NQ)
     D, F = lay.new_interior(NC), lay.new_interior(NC)
