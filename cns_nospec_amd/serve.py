@@ -155,7 +155,7 @@ class BatchServer:
     """Continuous batching on one BatchEngine."""
 
     def __init__(self, hbm_budget: int, fp32: bool = True, max_boxes: int = 4096,
-                 max_chunk: int = 16, keep_states=(), progress_every: int = 0,
+                 max_chunk: int = 8, keep_states=(), progress_every: int = 0,
                  max_resident_cells: int = 128 << 20, prepare_ahead: int = 16):
         from .batch import BatchEngine
         self.engine = BatchEngine(hbm_budget, fp32=fp32, max_boxes=max_boxes)
