@@ -26,9 +26,12 @@ def oracle_run(orc, edge, u0_compact, nsteps):
     return np.ascontiguousarray(lay.interior_of(lay.gview(U, NC))), dts
 
 
-@pytest.mark.parametrize("tiles", [None, {16: (64, 5), 20: (16, 7), 24: (32, 24), 36: (16, 8)}])
+@pytest.mark.parametrize("tiles", [None, {8: (32, 8), 13: (32, 5), 16: (64, 5), 20: (16, 7), 24: (32, 24),
+                                           36: (16, 8)}])
 def test_fp64_engine_bit_exact_vs_oracle(orc, device, tiles):
-    edges = (16, 20, 24, 36)
+    """The FP64 engine equals the oracle bit for bit on every box, down to the smallest
+    edge (8, where a 32-wide tile holds each cell four times over) and odd edges."""
+    edges = (8, 13, 16, 20, 24, 36)
     eng = BatchEngine(2 << 30, fp32=False, max_boxes=8, tuning=tiles or {})
     init = {e: compact_sinusoid(e, 50 + e, mach=0.2, kmax=3) for e in edges}
     ids = {e: eng.add(init[e].reshape(-1), e) for e in edges}
@@ -202,9 +205,9 @@ def test_engine_variants_agree(device, monkeypatch, fp32):
     (FP32 flux and FP64) against the one that gathers its halos, and the FP32 hypterm
     tile that keeps its z window's FP64 leading values for the centre plane against the
     one that loads them again."""
-    edges = (16, 20, 24, 40)
+    edges = (8, 13, 16, 20, 24, 40)
     init = {e: compact_sinusoid(e, 90 + e, mach=0.25, kmax=3) for e in edges}
-    tiles = {16: (16, 5), 20: (32, 7), 24: (64, 24), 40: (32, 16)}
+    tiles = {8: (32, 8), 13: (32, 5), 16: (16, 5), 20: (32, 7), 24: (64, 24), 40: (32, 16)}
     out = []
     for diff, keep, hyp in (("1", "0", "1"), ("1", "0", "0"), ("0", "0", "0"), ("1", "1", "0")):
         monkeypatch.setenv("CNS_BATCH_DIFF", diff)
