@@ -198,13 +198,21 @@ class BatchServer:
         while pending or waiting or active:
             arrivals()
             # admission: FIFO while the budget has room; each box stops by itself on the
-            # device once it has taken its requested steps
+            # device once it has taken its requested steps.  Initial states not built
+            # during the last chunk are built here with the GPU idle, so once boxes are
+            # resident at most `prepare_ahead` of them are built before the next chunk
+            # starts (a burst then ramps up in waves instead of waiting for all of them).
+            built = 0
             while waiting:
                 req = waiting[0]
                 if (self.max_resident_cells and active
                         and resident + req.edge ** 3 > self.max_resident_cells):
                     break
                 u0 = prepared.get(req.rid)
+                if u0 is None and active and built >= self.prepare_ahead:
+                    break
+                if u0 is None:
+                    built += 1
                 try:
                     bid = eng.add(u0 if u0 is not None else request_state(req).reshape(-1), req.edge)
                 except _lib.CnsBudgetError:

@@ -111,3 +111,35 @@ def test_batch_server_fifo_across_arrival_times(monkeypatch):
     assert len(admitted) == 12
     assert [r.arrival for r in sorted(out, key=lambda r: r.admitted)] == sorted(r.arrival for r in reqs)
     assert [r.rid for r in edges_order][:2] == [1, 0]  # the pair arriving first: longer one first
+
+
+def test_batch_server_burst_ramps_up_in_waves(monkeypatch):
+    """With every request waiting at t = 0, the server does not build all initial states
+    before the first chunk: at most `prepare_ahead` are built while no chunk runs, the
+    rest while chunks run, and every request is still served once, in order."""
+    from cns_nospec_amd import batch, serve
+    monkeypatch.setattr(batch, "BatchEngine", _FakeEngine)
+
+    def build(req):
+        _FakeEngine.log.append(("build", req.rid, req.edge))
+        return _Flat()
+
+    monkeypatch.setattr(serve, "request_state", build)
+    step = _FakeEngine.step
+
+    def logged_step(self, k):
+        _FakeEngine.log.append(("step", k, 0))
+        step(self, k)
+
+    monkeypatch.setattr(_FakeEngine, "step", logged_step)
+    reqs = make_stream(30, 0.0, 5)
+    srv = serve.BatchServer(10 ** 9, fp32=True, max_chunk=5, max_resident_cells=0, prepare_ahead=4)
+    out, summ = srv.run(reqs)
+    log = _FakeEngine.log
+    first_step = next(i for i, e in enumerate(log) if e[0] == "step")
+    assert sum(1 for e in log[:first_step] if e[0] == "build") == 4
+    assert sum(1 for e in log if e[0] == "build") == 30
+    assert summ["failed"] == 0 and summ["cell_steps"] == sum(r.edge ** 3 * r.steps for r in reqs)
+    order = [r.rid for r in sorted(reqs, key=lambda r: (r.arrival, -r.steps, r.rid))]
+    assert [e[1] for e in log if e[0] == "add"] == list(range(30))
+    assert [e[2] for e in log if e[0] == "add"] == [reqs[i].edge for i in order]
