@@ -994,6 +994,8 @@ struct HypDmaGeo : HypDmaBase {
     // buffer layout in bytes: 4 FP64 full tiles, then p and v full tiles, then the u strip
     static constexpr int BP = B64, BV = BP + TILE * FB, BU = BV + TILE * FB;
     static constexpr int BUF = BU + XSN * FB;                // 26880 (FP32) / 33280 (FP64) bytes
+    static_assert(TILE % EPS == 0 && XSN % EPS == 0, "whole wave-loads per flux-type region");
+    static_assert(XSN % 32 == 0 && BUF % 8 == 0, "whole FP64 wave-loads; buffers of whole doubles");
 };
 
 // Element of FP64 position slot s (lane pair L/2) in the full-tile layout.
