@@ -1,14 +1,14 @@
 // cns_batch.hip -- kernels of the batched engine: many independent periodic boxes
 // of uneven edges advanced together, a few launches per RK stage.
 //
-// Per stage, one kernel walks a tile work list (tiles of every active box).  For
-// each tile it does, in one pass and without ghost storage (halos are gathered
-// with periodic index wrap): diffterm and hypterm of the stage state, the SSP-RK3
-// update of each cell in FP64, and ctoprim of the updated cell, whose primitives
-// the next stage reads (so no separate ctoprim pass exists after start-up).  In
-// stage 3 it also folds the Courant number of U^{n+1} into the box's accumulator;
-// a small finalize kernel turns that into the next step's dt.  A step is thus four
-// launches and is captured once into a hipGraph.
+// Per stage and tile-width class, three kernels walk the tile work list (tiles of
+// every active box, one workgroup per tile), without ghost storage (halos are
+// loaded at periodically wrapped addresses): the wz field of the stage state,
+// diffterm, and hypterm with the SSP-RK3 update of each cell in FP64 and ctoprim of
+// the updated cell, whose primitives the next stage reads (so no separate ctoprim
+// pass exists after start-up).  In stage 3 the last also folds the Courant number
+// of U^{n+1} into the box's accumulator; a small finalize kernel turns that into
+// the next step's dt.  A step is captured once into a hipGraph.
 //
 // Flux arithmetic type FT: double reproduces the plain path (and the reference)
 // bit for bit; float is the FP32-flux mode, in which only the stencil arithmetic

@@ -5,7 +5,7 @@
 // does not fit is refused, never an out-of-memory fault) and nothing is
 // allocated while stepping.  Box records (BoxDev) live in a device array indexed
 // by box id.  The work list holds every resident box's tiles, grouped by tile-
-// width class; a step (3 RK stages x {diffterm, hypterm+update} per class, then
+// width class; a step (3 RK stages x {wz, diffterm, hypterm+update} per class, then
 // the dt finalize) is captured once into a hipGraph and replayed until the
 // resident set or the tile table changes.
 #include <algorithm>
