@@ -236,13 +236,27 @@ def quiet_stdout():
 
 
 def cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it (some hosts report only the brand, so the
+    family/model numbers and clock are added), and the logical CPUs this process may use."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                info.setdefault(k, v)
     except OSError:
+        return "unknown"
+    name = info.get("model name", "unknown")
+    extra = []
+    if "cpu family" in info and "model" in info:
+        extra.append(f"family {info['cpu family']} model {info['model']}")
+    if "cpu MHz" in info:
+        extra.append(f"{float(info['cpu MHz']) / 1e3:.2f} GHz")
+    try:
+        extra.append(f"{len(os.sched_getaffinity(0))} logical CPUs usable")
+    except (AttributeError, OSError):
         pass
-    return "unknown"
+    return f"{name} ({', '.join(extra)})" if extra else name
 
 
 def cpu_baseline_one(edge: int, budget_s: float):
