@@ -86,23 +86,39 @@ def request_state(req: Request, device="cuda"):
     """The request's initial ghost-free U (5, e, e, e) on the device.
 
     The five primitive fields are evaluated together (one set of launches for all
-    of them, no host synchronisation), mode by mode in the order the modes are drawn."""
+    of them), mode by mode in the order the modes are drawn.  Nothing in it waits for
+    the device: the per-field parameters reach it in one asynchronous copy from pinned
+    host memory (a copy from pageable memory would wait for all the work queued before
+    it, e.g. a chunk of steps that a server is running meanwhile)."""
     from .state import GAMMA, _draw_modes
     import torch
     e = req.edge
     rng = np.random.default_rng(req.seed)
     crd = (torch.arange(e, dtype=torch.float64, device=device) + 0.5) / e
     c0 = math.sqrt(GAMMA)
-    f0 = torch.tensor([1.0, 0.0, 0.0, 0.0, 1.0], dtype=torch.float64, device=device)
-    amp = torch.tensor([0.05] + [req.mach * c0] * 3 + [0.05], dtype=torch.float64, device=device)
     modes = [_draw_modes(rng, 2, 4) for _ in range(5)]  # rho, u, v, w, p
+    # rows: f0, amplitude, then per mode k (3 rows), phase (3 rows), mode amplitude
+    par = np.empty((2 + 4 * 7, 5))
+    par[0] = (1.0, 0.0, 0.0, 0.0, 1.0)
+    par[1] = (0.05, req.mach * c0, req.mach * c0, req.mach * c0, 0.05)
+    for m in range(4):
+        for f in range(5):
+            k, ph, a = modes[f][m]
+            par[2 + 7 * m:5 + 7 * m, f] = [int(k[d]) for d in range(3)]
+            par[5 + 7 * m:8 + 7 * m, f] = [float(ph[d]) for d in range(3)]
+            par[8 + 7 * m, f] = float(a)
+    if torch.device(device).type == "cuda":
+        host = torch.empty(par.shape, dtype=torch.float64, pin_memory=True)
+        host.copy_(torch.from_numpy(par))
+        dev = host.to(device, non_blocking=True)
+    else:
+        dev = torch.from_numpy(par).to(device)
+    f0, amp = dev[0], dev[1]
     s = torch.zeros((5, e, e, e), dtype=torch.float64, device=device)
     for m in range(4):
-        k = torch.tensor([[int(modes[f][m][0][d]) for d in range(3)] for f in range(5)],
-                         dtype=torch.float64, device=device)
-        ph = torch.tensor([[float(modes[f][m][1][d]) for d in range(3)] for f in range(5)],
-                          dtype=torch.float64, device=device)
-        a = torch.tensor([float(modes[f][m][2]) for f in range(5)], dtype=torch.float64, device=device)
+        k = dev[2 + 7 * m:5 + 7 * m].t()
+        ph = dev[5 + 7 * m:8 + 7 * m].t()
+        a = dev[8 + 7 * m]
         fx = torch.sin(2 * math.pi * k[:, 0:1] * crd + ph[:, 0:1])
         fy = torch.sin(2 * math.pi * k[:, 1:2] * crd + ph[:, 1:2])
         fz = torch.sin(2 * math.pi * k[:, 2:3] * crd + ph[:, 2:3])
