@@ -143,3 +143,23 @@ def test_batch_server_burst_ramps_up_in_waves(monkeypatch):
     order = [r.rid for r in sorted(reqs, key=lambda r: (r.arrival, -r.steps, r.rid))]
     assert [e[1] for e in log if e[0] == "add"] == list(range(30))
     assert [e[2] for e in log if e[0] == "add"] == [reqs[i].edge for i in order]
+
+
+def test_request_state_is_the_seeded_sinusoid():
+    """A request's initial state is the seeded sinusoidal state of the oracle's
+    generator (modes drawn field by field from the request's seed, wavenumbers 1-2,
+    amplitude Mach * c0 on the velocities): the server's device construction, run here
+    on the host, agrees with the NumPy one to rounding."""
+    import numpy as np
+    from cns_nospec_amd.serve import request_state
+    from cns_nospec_amd.state import conserved_from_primitives, sinusoid_primitives
+    for req in make_stream(4, 0.0, 11):
+        e = min(req.edge, 32)
+        req.edge = e
+        got = request_state(req, device="cpu").numpy()
+        prim = sinusoid_primitives(e, req.seed, mach=req.mach, kmax=2, nmodes=4)
+        ref = conserved_from_primitives(prim["rho"], prim["u"], prim["v"], prim["w"], prim["p"])
+        assert got.shape == (5, e, e, e)
+        for l in range(5):
+            scale = np.abs(ref[l]).max()
+            assert np.abs(got[l] - ref[l]).max() <= 1e-13 * scale, f"component {l}"
